@@ -1,0 +1,2 @@
+"""ORACLE package: CPU restatement of the reference BP, used only by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker."""
