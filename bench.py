@@ -1,0 +1,192 @@
+"""Benchmark: hierarchical min-sum BP on one MI355X.
+
+Workload (default): 1920x1080 grid, 128 labels, FP32 costs from the seeded
+synthetic stereo generator (seed 2, 64 rectangles, disparities 0..127,
+truncation 40), truncated-linear smoothness lambda=8 tau=4
+(useSmoothnessTruncLinear(8, 32)), 5 pyramid levels x 10 red-black
+half-sweeps per level, reference (mean) normalisation.  One step is one
+full decodeHbp on device-resident costs: relayout + pyramid + all sweeps +
+labelling.  Metric: node-label message updates per second (one update = one
+node x one label with its 4 outgoing messages), counted over every
+half-sweep of every level.
+
+Prints ONE JSON line.  Only one GPU: --gpus > 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E datasheet peak
+BYTES_PER_UPDATE = 36     # D (4) + 4 incoming (16) + 4 outgoing (16) FP32, per node-label
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(threads_all: int) -> dict:
+    """The oracle (plain-C restatement of the reference CPU BP) on a bounded
+    sample of the same workload: the config recipe at 480x270 (1/16 of the
+    pixels), 128 labels, 5 levels x 10 half-sweeps."""
+    from oracle import oracle as O
+    from gridinfcuda_amd import _lib as G
+    W, H, L = 480, 270, 128
+    data = G.gen_stereo(2, W, H, L, 64, 127, 40)["cost_f32"]
+    table = O.table_trunc_linear(L, 8, 32)
+    # node-label updates of the sample: same counting as the GPU metric
+    plan_u = 0.0
+    w, h = W, H
+    for t in range(5):
+        if t:
+            w, h = (w + 1) // 2, (h + 1) // 2
+        for i in range(10):
+            color = (i + 1) & 1
+            if w >= 3 and h >= 3:
+                ys, xs = np.mgrid[1:h - 1, 1:w - 1]
+                plan_u += float(((xs + ys) & 1 == color).sum()) * L
+    res = {}
+    for th in (1, threads_all):
+        t0 = time.perf_counter()
+        O.hbp(data, table, 2, 5, 10, norm=0, threads=th)
+        dt = time.perf_counter() - t0
+        res[th] = plan_u / dt / 1e9
+    return {
+        "value": res[1], "unit": "Gnode-label msg updates/s", "cores": 1, "kind": "port",
+        "sample": f"480x270x128 HBP 5 levels x 10 half-sweeps (config recipe at 1/16 of the pixels), "
+                  f"{plan_u:.4g} node-label updates, oracle built -O2 -m64",
+        "value_all_cores": res[threads_all], "cores_all": threads_all, "cpu_model": cpu_model(),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--labels", type=int, default=128)
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per node (0 = tuned default)")
+    ap.add_argument("--norm", type=int, default=0, help="0 mean (reference), 1 min")
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 chunked, 2 quad (per-direction lanes)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        return 2
+
+    import torch
+    from gridinfcuda_amd import _lib as G
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        return 1
+    W, H, L = args.width, args.height, args.labels
+    gen = G.gen_stereo(2, W, H, L, 64, 127, 40)["cost_f32"]
+    table = np.minimum(8 * np.abs(np.arange(L)[:, None] - np.arange(L)[None, :]), 32).astype(np.float32)
+    desc = G.make_desc(W, H, L, G.F32, G.SM_TRUNC_LINEAR, args.levels, args.iters, args.norm, args.lanes,
+                       args.kernel)
+    plan = G.Plan(desc, table)
+    data = torch.from_numpy(gen).cuda()
+    del gen
+    labels = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    for _ in range(args.warmup):
+        plan.run(data.data_ptr(), labels.data_ptr(), sp)
+    torch.cuda.synchronize()
+    plan.set_sweep_timing(True)
+    plan.sweep_stats(reset=True)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run(data.data_ptr(), labels.data_ptr(), sp)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sweep_ms, launches, sweep_updates = plan.sweep_stats(reset=True)
+    plan.set_sweep_timing(False)
+
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    U = plan.updates_per_run()
+    value = U / (ms_per_step / 1000.0) / 1e9
+
+    avg_launch_ms = sweep_ms / max(1, launches)
+    bytes_per_launch = BYTES_PER_UPDATE * sweep_updates / max(1, launches)
+    achieved = bytes_per_launch / (avg_launch_ms / 1000.0) / 1e9
+
+    # FP64 energy of the result (outside the timed region).
+    tdev = torch.from_numpy(table).cuda()
+    energy = G.energy_f64(W, H, L, G.F32, data.data_ptr(), tdev.data_ptr(), labels.data_ptr(), sp)
+
+    shape = "kernel=%s lanes=%d" % (["auto", "chunked", "quad"][args.kernel], args.lanes)
+    out = {
+        "metric": "Gnode-label msg updates/s",
+        "value": round(value, 3),
+        "unit": "Gnode-label msg updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded stereo generator)",
+        "config": {
+            "workload": f"hierarchical min-sum BP {W}x{H}, {L} labels, TL lambda=8 tau=4, "
+                        f"{args.levels} levels x {args.iters} half-sweeps, "
+                        f"{'mean' if args.norm == 0 else 'min'} normalisation",
+            "width": W, "height": H, "labels": L, "levels": args.levels, "iters_per_level": args.iters,
+            "node_label_updates_per_step": U, "tile": shape,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "k_sweep_linear steady-state instance (half-sweep)",
+            "bytes_per_update": BYTES_PER_UPDATE,
+            "avg_launch_ms": round(avg_launch_ms, 5),
+            "launches_timed": launches,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "energy_f64": energy,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

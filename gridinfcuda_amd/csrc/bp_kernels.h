@@ -1,0 +1,69 @@
+// Host-side launchers of the BP kernels (bp_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bp_layout.h"
+
+namespace ginf {
+
+// Register-tile shape of a sweep.
+//   chunked (default): C labels per lane, G lanes per node split the labels
+//   quad:    four lanes per node, one per direction, C = label capacity; the
+//            state uses the one-lane-per-node layout (layout_g() == 1)
+//   general: the O(L^2) kernel, one lane per node, C >= L
+struct SweepShape {
+    int C;
+    int G;
+    bool general;
+    bool quad;
+    int layout_g() const { return quad ? 1 : G; }
+};
+
+// Label capacities built for the quad kernel.
+#define GINF_QUAD_LIST(X) X(32) X(64) X(96) X(128) X(256)
+
+// Every (C, G) pair built for the O(L) kernel; L <= C * G.
+#define GINF_SHAPE_LIST(X) \
+    X(16, 1)               \
+    X(8, 2)                \
+    X(16, 2)               \
+    X(8, 4)                \
+    X(16, 4)               \
+    X(8, 8)                \
+    X(24, 4)               \
+    X(12, 8)               \
+    X(32, 4)               \
+    X(16, 8)               \
+    X(8, 16)               \
+    X(16, 16)
+
+struct UpsampleArgs {
+    const void *coarse[2][kNumDir];
+    void *fine[2][kNumDir];
+    LevelGeom cg, fg;
+    int C, G;
+};
+
+hipError_t launch_sweep(int dtype, const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
+                        hipStream_t st);
+// Per-family launchers (sweep_chunked_*.hip, sweep_quad.hip, sweep_general.hip).
+hipError_t launch_sweep_chunked_f32(const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
+                                    hipStream_t st);
+hipError_t launch_sweep_chunked_i32(const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
+                                    hipStream_t st);
+hipError_t launch_sweep_quad(int dtype, const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
+                             hipStream_t st);
+hipError_t launch_sweep_general(int dtype, const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
+                                hipStream_t st);
+hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, int C, int G, const LevelGeom &g,
+                           void *dst0, void *dst1, hipStream_t st);
+hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
+                          const LevelGeom &cg, int C, int G, hipStream_t st);
+hipError_t launch_upsample(int dtype, const UpsampleArgs &u, hipStream_t st);
+hipError_t launch_fill_u32(void *p, int64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_zero_labels(int32_t *labels, int W, int H, int color, hipStream_t st);
+hipError_t launch_energy(int dtype, const void *data, const void *table, const int32_t *labels, int W, int H, int L,
+                         double *partials, int nblocks, double *out, hipStream_t st);
+
+}  // namespace ginf

@@ -1,0 +1,201 @@
+// HIP kernels (gfx950) around the half-sweeps: data-cost relayout and
+// pyramid, explicit outbox upsample, fills, labels and the FP64 energy, plus
+// the half-sweep dispatch (kernels in sweep_*.hip).  Layout: see bp_layout.h.
+#include <hip/hip_runtime.h>
+
+#include "bp_kernels.h"
+
+namespace ginf {
+namespace {
+
+// Reference-layout costs [l][y][x] -> level-0 device layout, both colours.
+// Labels >= L (padding up to C*G) get cost 0; they are masked in the sweeps.
+template <typename T>
+__global__ __launch_bounds__(256) void k_relayout(const T *__restrict__ src, int W, int H, int L, int C, int G,
+                                                  LevelGeom g, T *dst0, T *dst1) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    T *dst = ((x + y) & 1) ? dst1 : dst0;
+    const int64_t slot = (int64_t)y * g.whp + (x >> 1);
+    const int LP = C * G;
+    for (int l = 0; l < LP; ++l) {
+        const int qq = l / C, i = l - qq * C;
+        const T v = (l < L) ? src[((int64_t)l * H + y) * W + x] : T(0);
+        dst[((int64_t)i * g.nplane + slot) * G + qq] = v;
+    }
+}
+
+// Data-cost pyramid level (src/bp.cc:100-117): parent (X,Y) sums its children
+// (2X,2Y), (2X+1,2Y), (2X,2Y+1), (2X+1,2Y+1) in that order, starting from 0.
+// Children (2X,2Y) and (2X+1,2Y+1) have colour 0, the other two colour 1, and
+// all four sit in slot column X of their rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pyramid(const T *f0, const T *f1, LevelGeom fg, T *c0, T *c1,
+                                                 LevelGeom cg, int C, int G) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int node = gid / G, q = gid - (gid / G) * G;
+    const int X = node % cg.w, Y = node / cg.w;
+    if (Y >= cg.h) return;
+    const bool ex = 2 * X + 1 < fg.w, ey = 2 * Y + 1 < fg.h;
+    const int64_t s00 = (int64_t)(2 * Y) * fg.whp + X;
+    const int64_t s01 = (int64_t)(2 * Y + 1) * fg.whp + X;
+    T *dst = ((X + Y) & 1) ? c1 : c0;
+    const int64_t cs = (int64_t)Y * cg.whp + (X >> 1);
+    for (int i = 0; i < C; ++i) {
+        const int64_t fo = (int64_t)i * fg.nplane;
+        T acc = T(0);
+        acc = acc + f0[(fo + s00) * G + q];
+        if (ex) acc = acc + f1[(fo + s00) * G + q];
+        if (ey) acc = acc + f1[(fo + s01) * G + q];
+        if (ex && ey) acc = acc + f0[(fo + s01) * G + q];
+        dst[((int64_t)i * cg.nplane + cs) * G + q] = acc;
+    }
+}
+
+// Explicit outbox upsample (src/bp.cc:129-139), every fine node including borders.
+template <typename T>
+__global__ __launch_bounds__(256) void k_upsample(UpsampleArgs u) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int G = u.G;
+    const int node = gid / G, q = gid - (gid / G) * G;
+    const int x = node % u.fg.w, y = node / u.fg.w;
+    if (y >= u.fg.h) return;
+    const int px = x >> 1, py = y >> 1;
+    const int fc = (x + y) & 1, pc = (px + py) & 1;
+    const int64_t fs = (int64_t)y * u.fg.whp + (x >> 1);
+    const int64_t ps = (int64_t)py * u.cg.whp + (px >> 1);
+    for (int d = 0; d < 4; ++d) {
+        const T *src = (const T *)u.coarse[pc][d];
+        T *dst = (T *)u.fine[fc][d];
+        for (int i = 0; i < u.C; ++i)
+            dst[((int64_t)i * u.fg.nplane + fs) * G + q] = src[((int64_t)i * u.cg.nplane + ps) * G + q];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Interior nodes of colour `color` get label 0 (their belief stayed zero).
+__global__ __launch_bounds__(256) void k_zero_labels(int32_t *labels, int W, int H, int color) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x < 1 || x > W - 2 || y < 1 || y > H - 2) return;
+    if (((x + y) & 1) == color) labels[(int64_t)y * W + x] = 0;
+}
+
+// FP64 energy: per-block partial sums over interior nodes in row-major order,
+// then one block adds the partials in index order (deterministic).
+template <typename T>
+__global__ __launch_bounds__(256) void k_energy_partial(const T *data, const T *table, const int32_t *f, int W,
+                                                        int H, int L, double *partials) {
+    __shared__ double red[256];
+    const int64_t nin = (int64_t)(W - 2) * (H - 2);
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nin; i += (int64_t)gridDim.x * blockDim.x) {
+        const int y = 1 + (int)(i / (W - 2));
+        const int x = 1 + (int)(i - (int64_t)(y - 1) * (W - 2));
+        const int fl = f[(int64_t)y * W + x];
+        const int fn = f[(int64_t)(y - 1) * W + x];
+        const int fe = f[(int64_t)y * W + x + 1];
+        if ((unsigned)fl >= (unsigned)L || (unsigned)fn >= (unsigned)L || (unsigned)fe >= (unsigned)L) {
+            acc = __builtin_nan("");  // label out of range: reported as an error by the host
+            continue;
+        }
+        acc += (double)data[((int64_t)fl * H + y) * W + x];
+        acc += (double)table[fl + L * fn];
+        acc += (double)table[fl + L * fe];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_energy_final(const double *partials, int n, double *out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += partials[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+}  // namespace
+
+hipError_t launch_sweep(int dtype, const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
+                        hipStream_t st) {
+    if (sh.quad) return launch_sweep_quad(dtype, sh, a, zeroin, labels, st);
+    if (sh.general) return launch_sweep_general(dtype, sh, a, zeroin, labels, st);
+    return dtype == 0 ? launch_sweep_chunked_f32(sh, a, zeroin, labels, st)
+                      : launch_sweep_chunked_i32(sh, a, zeroin, labels, st);
+}
+hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, int C, int G, const LevelGeom &g,
+                           void *dst0, void *dst1, hipStream_t st) {
+    const dim3 grid((unsigned)((W + 255) / 256), (unsigned)H), block(256);
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_relayout<float>, grid, block, 0, st, (const float *)src, W, H, L, C, G, g,
+                           (float *)dst0, (float *)dst1);
+    else
+        hipLaunchKernelGGL(k_relayout<int>, grid, block, 0, st, (const int *)src, W, H, L, C, G, g, (int *)dst0,
+                           (int *)dst1);
+    return hipGetLastError();
+}
+
+hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
+                          const LevelGeom &cg, int C, int G, hipStream_t st) {
+    const int64_t threads = (int64_t)cg.w * cg.h * G;
+    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_pyramid<float>, grid, block, 0, st, (const float *)f0, (const float *)f1, fg,
+                           (float *)c0, (float *)c1, cg, C, G);
+    else
+        hipLaunchKernelGGL(k_pyramid<int>, grid, block, 0, st, (const int *)f0, (const int *)f1, fg, (int *)c0,
+                           (int *)c1, cg, C, G);
+    return hipGetLastError();
+}
+
+hipError_t launch_upsample(int dtype, const UpsampleArgs &u, hipStream_t st) {
+    const int64_t threads = (int64_t)u.fg.w * u.fg.h * u.G;
+    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+    if (dtype == 0) hipLaunchKernelGGL(k_upsample<float>, grid, block, 0, st, u);
+    else hipLaunchKernelGGL(k_upsample<int>, grid, block, 0, st, u);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(void *p, int64_t n, uint32_t v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, st, (uint32_t *)p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_labels(int32_t *labels, int W, int H, int color, hipStream_t st) {
+    const dim3 grid((unsigned)((W + 255) / 256), (unsigned)H), block(256);
+    hipLaunchKernelGGL(k_zero_labels, grid, block, 0, st, labels, W, H, color);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(int dtype, const void *data, const void *table, const int32_t *labels, int W, int H, int L,
+                         double *partials, int nblocks, double *out, hipStream_t st) {
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_energy_partial<float>, dim3(nblocks), dim3(256), 0, st, (const float *)data,
+                           (const float *)table, labels, W, H, L, partials);
+    else
+        hipLaunchKernelGGL(k_energy_partial<int>, dim3(nblocks), dim3(256), 0, st, (const int *)data,
+                           (const int *)table, labels, W, H, L, partials);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, partials, nblocks, out);
+    return hipGetLastError();
+}
+
+}  // namespace ginf

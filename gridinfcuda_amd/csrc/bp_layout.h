@@ -1,0 +1,53 @@
+// Device-side data layout of the grid BP state and the kernel argument blocks.
+//
+// A level of w x h nodes is split by colour c = (x + y) & 1 (the red-black
+// classes of the reference schedule, src/bp.cc:149-150).  Node (x, y) of
+// colour c lives in slot n = y * whp + (x >> 1) of that colour's arrays,
+// where whp >= ceil(w / 2) is the padded half-row.  A node's L labels are
+// spread over G lanes ("lanes per node"): lane q owns labels q*C .. q*C+C-1,
+// and label l = q*C + i of slot n is element (i * nplane + n) * G + q.
+// With this layout one wavefront reading register i of 64/G consecutive
+// nodes touches 64 consecutive elements (256 B) for every array.
+//
+// Per level and colour there is one data-cost array and four outbox arrays
+// (directions S, E, N, W as in src/include/utils.h:26-37).  Outbox (x,y,d) is
+// the message node (x,y) sends towards its neighbour in direction d.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ginf {
+
+constexpr int kNumDir = 4;
+constexpr int kSrcSame = 0;    // incoming messages from this level, other colour
+constexpr int kSrcCoarse = 1;  // incoming messages read through the parent at level+1
+
+struct LevelGeom {
+    int w, h;        // nodes
+    int whp;         // padded half-row (slots per row per colour)
+    int nslots;      // h * whp
+    int64_t nplane;  // slots per label-register plane (nslots rounded up)
+};
+
+// Arguments of one red-black half-sweep launch (linear and general models).
+struct SweepArgs {
+    const void *data;        // data costs of the moving colour
+    void *out[kNumDir];      // outboxes of the moving colour
+    const void *in[2][kNumDir];  // [colour][dir] outbox arrays incoming messages are read from
+    int64_t stride;          // elements between register planes of this level (nplane * G)
+    int64_t in_stride;       // same for the arrays in `in`
+    int w, h, whp, nslots;   // this level
+    int in_whp;              // half-row of the `in` level
+    int color;               // moving colour: nodes with (x + y) & 1 == color
+    int src;                 // kSrcSame / kSrcCoarse
+    int num_labels;          // L
+    int norm;                // 0 mean, 1 min
+    int32_t *labels;         // W x H result (level 0), written when the kernel is built for it
+    int label_w;
+    float s_f, tau_f;        // s = V(0,1), tau = V(0,L-1) (src/bp.cc:35,43)
+    int s_i, tau_i;
+    const void *table;       // L x L table (general model)
+};
+
+}  // namespace ginf

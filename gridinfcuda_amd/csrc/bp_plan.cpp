@@ -1,0 +1,468 @@
+// C ABI of the grid BP library: plans (device workspace per problem shape),
+// the HBP driver, the synchronous host-pointer entry, the FP64 energy and
+// test hooks.  The driver follows decodeHbp (reference src/bp.cc:86-180):
+// data-cost pyramid, then levels coarse to fine, each starting from the
+// outbox messages of the level above and running `iters` red-black
+// half-sweeps, then the stale-belief labelling of the finest level.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bp_kernels.h"
+#include "ginf_bp.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define GINF_HIP_TRY(expr)                                                                                   \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess)                                                                                \
+            return set_error(GINF_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                                                      \
+    } while (0)
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// Default register tile per label count for the O(L) kernel: the fewest
+// lanes per node that keep C <= 16 labels per lane.
+bool pick_linear_shape(int L, int lanes, ginf::SweepShape &sh) {
+    struct P { int C, G; };
+    static const P pairs[] = {
+#define GINF_PAIR(CC, GG) {CC, GG},
+        GINF_SHAPE_LIST(GINF_PAIR)
+#undef GINF_PAIR
+    };
+    if (lanes == 0) {
+        int g = L <= 16 ? 1 : L <= 32 ? 2 : L <= 64 ? 4 : L <= 128 ? 8 : L <= 256 ? 16 : 0;
+        if (g == 0) return false;
+        if (L > 64 && L <= 96) { sh = {12, 8, false, false}; return true; }
+        sh = {16, g, false, false};
+        return true;
+    }
+    int best = -1;
+    for (int i = 0; i < (int)(sizeof pairs / sizeof pairs[0]); ++i)
+        if (pairs[i].G == lanes && pairs[i].C * pairs[i].G >= L && (best < 0 || pairs[i].C < pairs[best].C)) best = i;
+    if (best < 0) return false;
+    sh = {pairs[best].C, pairs[best].G, false, false};
+    return true;
+}
+
+bool pick_quad_shape(int L, ginf::SweepShape &sh) {
+    static const int caps[] = {
+#define GINF_CAP(LM) LM,
+        GINF_QUAD_LIST(GINF_CAP)
+#undef GINF_CAP
+    };
+    for (int c : caps)
+        if (L <= c) { sh = {c, 4, false, true}; return true; }
+    return false;
+}
+
+bool pick_general_shape(int L, int lanes, ginf::SweepShape &sh) {
+    if (lanes > 1) return false;
+    for (int c : {16, 32, 64, 128})
+        if (L <= c) { sh = {c, 1, true, false}; return true; }
+    return false;
+}
+
+}  // namespace
+
+struct ginf_bp_plan_s {
+    ginf_bp_desc desc{};
+    ginf::SweepShape shape{};
+    int levels = 0;
+    std::vector<ginf::LevelGeom> geom;
+    // per level: data[c], msgs[c][d]
+    struct Level { void *data[2]; void *msgs[2][4]; };
+    std::vector<Level> lv;
+    void *pool = nullptr;
+    size_t pool_bytes = 0;
+    void *table_dev = nullptr;
+    std::vector<unsigned char> table_host;
+    float s_f = 0, tau_f = 0;
+    int s_i = 0, tau_i = 0;
+    int device = 0;
+    // sweep timing (profiling hook)
+    bool timing = false;
+    struct Timed { hipEvent_t beg, end; double updates; };
+    std::vector<Timed> timed;
+    std::vector<hipEvent_t> spare;
+};
+
+namespace {
+
+size_t elem_size(int dtype) { (void)dtype; return 4; }
+
+int validate_desc(const ginf_bp_desc *d) {
+    if (!d) return set_error(GINF_ERR_INVALID, "desc is NULL");
+    if (d->width < 1 || d->height < 1) return set_error(GINF_ERR_INVALID, "grid %dx%d must be at least 1x1", d->width, d->height);
+    if (d->num_labels < 1) return set_error(GINF_ERR_INVALID, "num_labels %d < 1", d->num_labels);
+    if (d->dtype != GINF_F32 && d->dtype != GINF_I32) return set_error(GINF_ERR_INVALID, "dtype %d", d->dtype);
+    if (d->sm_model < GINF_SMOOTH_EXPLICIT || d->sm_model > GINF_SMOOTH_TRUNC_QUAD)
+        return set_error(GINF_ERR_INVALID, "sm_model %d", d->sm_model);
+    if (d->num_levels < 1) return set_error(GINF_ERR_INVALID, "num_levels %d < 1", d->num_levels);
+    if (d->iters_per_level < 0) return set_error(GINF_ERR_INVALID, "iters_per_level %d < 0", d->iters_per_level);
+    if (d->norm != GINF_NORM_MEAN && d->norm != GINF_NORM_MIN) return set_error(GINF_ERR_INVALID, "norm %d", d->norm);
+    if (d->kernel < GINF_KERNEL_AUTO || d->kernel > GINF_KERNEL_QUAD) return set_error(GINF_ERR_INVALID, "kernel %d", d->kernel);
+    if (d->lanes_per_node < 0 || d->lanes_per_node > 16 || (d->lanes_per_node & (d->lanes_per_node - 1)))
+        return set_error(GINF_ERR_INVALID, "lanes_per_node %d (0, 1, 2, 4, 8 or 16)", d->lanes_per_node);
+    for (int r : d->reserved)
+        if (r != 0) return set_error(GINF_ERR_INVALID, "reserved fields must be zero");
+    if ((int64_t)d->width * d->height > (int64_t)1 << 30)
+        return set_error(GINF_ERR_UNSUPPORTED, "grid of %lld nodes is too large", (long long)d->width * d->height);
+    return GINF_OK;
+}
+
+bool is_linear(int model) { return model == GINF_SMOOTH_TRUNC_LINEAR || model == GINF_SMOOTH_POTTS; }
+
+// Interior nodes (1 <= x <= w-2, 1 <= y <= h-2) with (x + y) & 1 == color.
+int64_t interior_of_color(int w, int h, int color) {
+    if (w < 3 || h < 3) return 0;
+    int64_t n = 0;
+    for (int y = 1; y <= h - 2; ++y) {
+        const int x0 = ((y + color) & 1) ? 1 : 2;  // first x >= 1 with (x + y) & 1 == color
+        if (x0 <= w - 2) n += (w - 2 - x0) / 2 + 1;
+    }
+    return n;
+}
+
+hipEvent_t take_event(ginf_bp_plan p) {
+    if (!p->spare.empty()) { hipEvent_t e = p->spare.back(); p->spare.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int plan_run(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStream_t st) {
+    using namespace ginf;
+    const ginf_bp_desc &d = p->desc;
+    const int dt = d.dtype, C = p->shape.C, G = p->shape.layout_g(), T = p->levels, N = d.iters_per_level;
+    const int W = d.width, H = d.height, L = d.num_labels;
+    GINF_HIP_TRY(launch_relayout(dt, data_dev, W, H, L, C, G, p->geom[0], p->lv[0].data[0], p->lv[0].data[1], st));
+    for (int t = 1; t < T; ++t)
+        GINF_HIP_TRY(launch_pyramid(dt, p->lv[t - 1].data[0], p->lv[t - 1].data[1], p->geom[t - 1], p->lv[t].data[0],
+                                    p->lv[t].data[1], p->geom[t], C, G, st));
+    if (N == 0) {
+        // Every belief stays zero: interior labels are 0 (src/bp.cc:168-173).
+        GINF_HIP_TRY(launch_zero_labels(labels_dev, W, H, 0, st));
+        GINF_HIP_TRY(launch_zero_labels(labels_dev, W, H, 1, st));
+        return GINF_OK;
+    }
+    // With two or more half-sweeps per level both colours move, so the first
+    // half-sweep of a level can read the level above directly (fused
+    // upsample), and the top level's first half-sweep reads zero messages.
+    const bool fused = N >= 2;
+    for (int t = T - 1; t >= 0; --t) {
+        const LevelGeom &g = p->geom[t];
+        const int64_t per = (int64_t)C * g.nplane * G;
+        if (!fused) {
+            if (t == T - 1) {
+                for (int c = 0; c < 2; ++c)
+                    for (int k = 0; k < 4; ++k) GINF_HIP_TRY(launch_fill_u32(p->lv[t].msgs[c][k], per, 0u, st));
+            } else {
+                UpsampleArgs u{};
+                for (int c = 0; c < 2; ++c)
+                    for (int k = 0; k < 4; ++k) { u.coarse[c][k] = p->lv[t + 1].msgs[c][k]; u.fine[c][k] = p->lv[t].msgs[c][k]; }
+                u.cg = p->geom[t + 1];
+                u.fg = g;
+                u.C = C;
+                u.G = G;
+                GINF_HIP_TRY(launch_upsample(dt, u, st));
+            }
+        }
+        for (int i = 0; i < N; ++i) {
+            SweepArgs a{};
+            const int color = (i + 1) & 1;  // (x + y + i) odd (src/bp.cc:150)
+            a.data = p->lv[t].data[color];
+            for (int k = 0; k < 4; ++k) a.out[k] = p->lv[t].msgs[color][k];
+            bool zeroin = false;
+            const int src_level = (fused && i == 0 && t < T - 1) ? t + 1 : t;
+            if (fused && i == 0 && t == T - 1) zeroin = true;
+            for (int c = 0; c < 2; ++c)
+                for (int k = 0; k < 4; ++k) a.in[c][k] = p->lv[src_level].msgs[c][k];
+            a.src = src_level == t ? kSrcSame : kSrcCoarse;
+            a.stride = g.nplane * G;
+            a.in_stride = p->geom[src_level].nplane * G;
+            a.w = g.w; a.h = g.h; a.whp = g.whp; a.nslots = g.nslots;
+            a.in_whp = p->geom[src_level].whp;
+            a.color = color;
+            a.num_labels = L;
+            a.norm = d.norm;
+            a.labels = labels_dev;
+            a.label_w = W;
+            a.s_f = p->s_f; a.tau_f = p->tau_f; a.s_i = p->s_i; a.tau_i = p->tau_i;
+            a.table = p->table_dev;
+            const bool labels = (t == 0) && (i >= N - 2);
+            ginf_bp_plan_s::Timed tm{nullptr, nullptr, 0.0};
+            // Only the steady-state kernel instance (no zero-input, no labels) is
+            // timed, so the figure matches a per-kernel profiler summary.
+            if (p->timing && !zeroin && !labels) {
+                tm.beg = take_event(p);
+                tm.end = take_event(p);
+                tm.updates = (double)interior_of_color(g.w, g.h, color) * L;
+                if (tm.beg) GINF_HIP_TRY(hipEventRecord(tm.beg, st));
+            }
+            GINF_HIP_TRY(launch_sweep(dt, p->shape, a, zeroin, labels, st));
+            if (tm.beg && tm.end) {
+                GINF_HIP_TRY(hipEventRecord(tm.end, st));
+                p->timed.push_back(tm);
+            }
+        }
+        if (t == 0 && N == 1) GINF_HIP_TRY(launch_zero_labels(labels_dev, W, H, 0, st));
+    }
+    return GINF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ginf_last_error(void) { return g_last_error.c_str(); }
+
+const char *ginf_version(void) { return "gridinfcuda_amd 0.1 (gfx950)"; }
+
+int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, ginf_bp_plan *out) {
+    if (!out) return set_error(GINF_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate_desc(desc);
+    if (rc) return rc;
+    if (!sm_table_host) return set_error(GINF_ERR_INVALID, "sm_table_host is NULL");
+    const int L = desc->num_labels;
+    ginf::SweepShape sh{};
+    bool ok;
+    if (!is_linear(desc->sm_model)) {
+        ok = desc->kernel != GINF_KERNEL_QUAD && pick_general_shape(L, desc->lanes_per_node, sh);
+    } else if (desc->kernel == GINF_KERNEL_QUAD) {
+        ok = pick_quad_shape(L, sh);
+    } else if (desc->kernel == GINF_KERNEL_CHUNKED || desc->lanes_per_node != 0) {
+        ok = pick_linear_shape(L, desc->lanes_per_node, sh);
+    } else if (desc->dtype == GINF_F32 && desc->norm == GINF_NORM_MEAN && L > 16) {
+        // FP32 mean normalisation rounds differently under any other order:
+        // keep the reference's sequential per-direction order.
+        ok = pick_quad_shape(L, sh);
+    } else {
+        ok = pick_linear_shape(L, 0, sh);
+    }
+    if (!ok)
+        return set_error(GINF_ERR_UNSUPPORTED, "no kernel for L=%d, model %d, lanes_per_node %d", L, desc->sm_model,
+                         desc->lanes_per_node);
+    ginf_bp_plan p = new (std::nothrow) ginf_bp_plan_s;
+    if (!p) return set_error(GINF_ERR_NOMEM, "host allocation failed");
+    p->desc = *desc;
+    p->shape = sh;
+    p->levels = desc->num_levels;
+    hipGetDevice(&p->device);
+    const size_t es = elem_size(desc->dtype);
+    p->table_host.resize((size_t)L * L * es);
+    memcpy(p->table_host.data(), sm_table_host, p->table_host.size());
+    // O(L) parameters s = V(0,1) and tau = V(0,L-1) (src/bp.cc:35,43); with
+    // L == 1 the scans are empty and s is unused.
+    if (desc->dtype == GINF_F32) {
+        const float *tb = (const float *)sm_table_host;
+        p->s_f = L > 1 ? tb[L] : 0.f;
+        p->tau_f = tb[(size_t)L * (L - 1)];
+    } else {
+        const int *tb = (const int *)sm_table_host;
+        p->s_i = L > 1 ? tb[L] : 0;
+        p->tau_i = tb[(size_t)L * (L - 1)];
+    }
+    // Level geometry: ceil halving (src/bp.cc:101-102).
+    int w = desc->width, h = desc->height;
+    size_t total = 0;
+    std::vector<size_t> offs;
+    for (int t = 0; t < p->levels; ++t) {
+        if (t > 0) { w = (w + 1) / 2; h = (h + 1) / 2; }
+        ginf::LevelGeom g{};
+        g.w = w; g.h = h;
+        g.whp = (int)round_up((w + 1) / 2, 16);
+        g.nslots = g.whp * h;
+        g.nplane = round_up(g.nslots, 64);
+        p->geom.push_back(g);
+        const size_t arr = (size_t)sh.C * g.nplane * sh.layout_g() * es;
+        for (int k = 0; k < 10; ++k) { offs.push_back(total); total += round_up(arr, 256); }
+    }
+    total += round_up((size_t)L * L * es, 256);
+    p->pool_bytes = total;
+    hipError_t e = hipMalloc(&p->pool, total);
+    if (e != hipSuccess) {
+        delete p;
+        return set_error(GINF_ERR_NOMEM, "hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
+    }
+    // Zero everything once: border outboxes are never written and stay 0.
+    e = hipMemset(p->pool, 0, total);
+    if (e == hipSuccess) {
+        char *base = (char *)p->pool;
+        p->lv.resize(p->levels);
+        size_t k = 0;
+        for (int t = 0; t < p->levels; ++t) {
+            for (int c = 0; c < 2; ++c) p->lv[t].data[c] = base + offs[k++];
+            for (int c = 0; c < 2; ++c)
+                for (int dd = 0; dd < 4; ++dd) p->lv[t].msgs[c][dd] = base + offs[k++];
+        }
+        p->table_dev = base + total - round_up((size_t)L * L * es, 256);
+        e = hipMemcpy(p->table_dev, sm_table_host, (size_t)L * L * es, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        hipFree(p->pool);
+        delete p;
+        return set_error(GINF_ERR_HIP, "plan set-up failed: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return GINF_OK;
+}
+
+int ginf_bp_plan_destroy(ginf_bp_plan plan) {
+    if (!plan) return GINF_OK;
+    for (auto &t : plan->timed) { if (t.beg) hipEventDestroy(t.beg); if (t.end) hipEventDestroy(t.end); }
+    for (auto e : plan->spare) hipEventDestroy(e);
+    if (plan->pool) hipFree(plan->pool);
+    delete plan;
+    return GINF_OK;
+}
+
+int ginf_bp_plan_workspace_bytes(ginf_bp_plan plan, size_t *bytes) {
+    if (!plan || !bytes) return set_error(GINF_ERR_INVALID, "NULL argument");
+    *bytes = plan->pool_bytes;
+    return GINF_OK;
+}
+
+int ginf_bp_plan_set_sweep_timing(ginf_bp_plan plan, int enable) {
+    if (!plan) return set_error(GINF_ERR_INVALID, "NULL plan");
+    plan->timing = enable != 0;
+    return GINF_OK;
+}
+
+int ginf_bp_plan_sweep_stats(ginf_bp_plan p, double *total_ms, int64_t *launches, double *updates, int reset) {
+    if (!p || !total_ms || !launches || !updates) return set_error(GINF_ERR_INVALID, "NULL argument");
+    double ms = 0.0, up = 0.0;
+    for (auto &t : p->timed) {
+        GINF_HIP_TRY(hipEventSynchronize(t.end));
+        float e = 0.f;
+        GINF_HIP_TRY(hipEventElapsedTime(&e, t.beg, t.end));
+        ms += e;
+        up += t.updates;
+    }
+    *total_ms = ms;
+    *launches = (int64_t)p->timed.size();
+    *updates = up;
+    if (reset) {
+        for (auto &t : p->timed) { p->spare.push_back(t.beg); p->spare.push_back(t.end); }
+        p->timed.clear();
+    }
+    return GINF_OK;
+}
+
+int ginf_bp_plan_updates_per_run(ginf_bp_plan p, double *updates) {
+    if (!p || !updates) return set_error(GINF_ERR_INVALID, "NULL argument");
+    double u = 0.0;
+    const int N = p->desc.iters_per_level;
+    for (int t = 0; t < p->levels; ++t)
+        for (int i = 0; i < N; ++i)
+            u += (double)interior_of_color(p->geom[t].w, p->geom[t].h, (i + 1) & 1) * p->desc.num_labels;
+    *updates = u;
+    return GINF_OK;
+}
+
+int ginf_bp_plan_level_dims(ginf_bp_plan plan, int level, int *w, int *h) {
+    if (!plan || !w || !h) return set_error(GINF_ERR_INVALID, "NULL argument");
+    if (level < 0 || level >= plan->levels) return set_error(GINF_ERR_INVALID, "level %d out of range", level);
+    *w = plan->geom[level].w;
+    *h = plan->geom[level].h;
+    return GINF_OK;
+}
+
+int ginf_bp_plan_run(ginf_bp_plan plan, const void *data_dev, int32_t *labels_dev, void *stream) {
+    if (!plan || !data_dev || !labels_dev) return set_error(GINF_ERR_INVALID, "NULL argument");
+    return plan_run(plan, data_dev, labels_dev, (hipStream_t)stream);
+}
+
+int ginf_bp_plan_dump_msgs(ginf_bp_plan p, int level, void *msgs_host) {
+    if (!p || !msgs_host) return set_error(GINF_ERR_INVALID, "NULL argument");
+    if (level < 0 || level >= p->levels) return set_error(GINF_ERR_INVALID, "level %d out of range", level);
+    const ginf::LevelGeom &g = p->geom[level];
+    const int C = p->shape.C, G = p->shape.layout_g(), L = p->desc.num_labels;
+    const size_t n = (size_t)C * g.nplane * G;
+    std::vector<uint32_t> buf(n);
+    uint32_t *dst = (uint32_t *)msgs_host;
+    GINF_HIP_TRY(hipDeviceSynchronize());
+    for (int c = 0; c < 2; ++c)
+        for (int d = 0; d < 4; ++d) {
+            GINF_HIP_TRY(hipMemcpy(buf.data(), p->lv[level].msgs[c][d], n * 4, hipMemcpyDeviceToHost));
+            for (int y = 0; y < g.h; ++y)
+                for (int x = 0; x < g.w; ++x) {
+                    if (((x + y) & 1) != c) continue;
+                    const size_t slot = (size_t)y * g.whp + (x >> 1);
+                    for (int l = 0; l < L; ++l) {
+                        const int q = l / C, i = l - q * C;
+                        dst[(size_t)x + (size_t)g.w * ((size_t)y + (size_t)g.h * ((size_t)d + 4u * l))] =
+                            buf[((size_t)i * g.nplane + slot) * G + q];
+                    }
+                }
+        }
+    return GINF_OK;
+}
+
+int ginf_hbp_host(const ginf_bp_desc *desc, const void *data_host, const void *sm_table_host, int32_t *labels_host) {
+    int rc = validate_desc(desc);
+    if (rc) return rc;
+    if (!data_host || !labels_host) return set_error(GINF_ERR_INVALID, "NULL argument");
+    ginf_bp_plan p = nullptr;
+    rc = ginf_bp_plan_create(desc, sm_table_host, &p);
+    if (rc) return rc;
+    const size_t nd = (size_t)desc->width * desc->height * desc->num_labels * 4;
+    const size_t nl = (size_t)desc->width * desc->height * 4;
+    void *dd = nullptr, *dl = nullptr;
+    hipError_t e = hipMalloc(&dd, nd);
+    if (e == hipSuccess) e = hipMalloc(&dl, nl);
+    if (e == hipSuccess) e = hipMemcpy(dd, data_host, nd, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dl, labels_host, nl, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rc = plan_run(p, dd, (int32_t *)dl, nullptr);
+        if (rc == GINF_OK) e = hipMemcpy(labels_host, dl, nl, hipMemcpyDeviceToHost);
+    }
+    if (dd) hipFree(dd);
+    if (dl) hipFree(dl);
+    ginf_bp_plan_destroy(p);
+    if (e != hipSuccess) return set_error(GINF_ERR_HIP, "ginf_hbp_host: %s", hipGetErrorString(e));
+    return rc;
+}
+
+int ginf_energy_f64(int width, int height, int num_labels, int dtype, const void *data_dev, const void *table_dev,
+                    const int32_t *labels_dev, double *energy_host, void *stream) {
+    if (!data_dev || !table_dev || !labels_dev || !energy_host) return set_error(GINF_ERR_INVALID, "NULL argument");
+    if (width < 1 || height < 1 || num_labels < 1) return set_error(GINF_ERR_INVALID, "bad sizes");
+    if (dtype != GINF_F32 && dtype != GINF_I32) return set_error(GINF_ERR_INVALID, "dtype %d", dtype);
+    if (width < 3 || height < 3) { *energy_host = 0.0; return GINF_OK; }
+    const int nblocks = 1024;
+    double *buf = nullptr;
+    GINF_HIP_TRY(hipMalloc(&buf, sizeof(double) * (nblocks + 1)));
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = ginf::launch_energy(dtype, data_dev, table_dev, labels_dev, width, height, num_labels, buf, nblocks,
+                                       buf + nblocks, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(energy_host, buf + nblocks, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(buf);
+    if (e != hipSuccess) return set_error(GINF_ERR_HIP, "energy: %s", hipGetErrorString(e));
+    if (*energy_host != *energy_host) return set_error(GINF_ERR_INVALID, "a label is outside [0, %d)", num_labels);
+    return GINF_OK;
+}
+
+}  // extern "C"

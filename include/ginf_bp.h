@@ -1,0 +1,141 @@
+/*
+ * ginf_bp.h -- C ABI of the MI355X (gfx950) grid min-sum belief propagation
+ * library.  Plain pointers and sizes only; no HIP or torch types cross it
+ * (streams are passed as void* holding a hipStream_t, NULL = null stream).
+ *
+ * Every function returns GINF_OK (0) or a positive GINF_ERR_* code and
+ * never throws; ginf_last_error() describes the last failure on the
+ * calling thread.
+ *
+ * The BP entry points replace the reference's
+ *   template<T> void decodeBp(Grid<T>*, int numIters, Matrix<int>*)          src/include/bp.h:13-14
+ *   template<T> void gpuDecodeBp(Grid<T>*, int numIters, Matrix<int>*)       src/include/bp.h:17-18
+ *   template<T> void decodeHbp(Grid<T>*, int levels, int iters, Matrix<int>*) src/include/bp.h:21-22
+ *   template<T> void gpuDecodeHbp(Grid<T>*, int levels, int iters, Matrix<int>*) src/include/bp.h:25-26
+ * implemented in src/bp.cc:86-187 and src/gpu/bp.cu:133-234; the energy
+ * replaces Grid<T>::getLabelingCost (src/grid.cc:102-125).  The C++ drop-in
+ * templates in bp.h/grid.h/matrix.h are thin wrappers over this ABI.
+ */
+#ifndef GINF_BP_H_C_ABI
+#define GINF_BP_H_C_ABI
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ginf_status {
+    GINF_OK = 0,
+    GINF_ERR_INVALID = 1,      /* bad argument (sizes, null pointers, enums) */
+    GINF_ERR_UNSUPPORTED = 2,  /* valid but not built (e.g. label count too large) */
+    GINF_ERR_HIP = 3,          /* HIP runtime failure (message in ginf_last_error) */
+    GINF_ERR_NOMEM = 4         /* device allocation failed */
+};
+
+/* Element type of costs and messages: the reference instantiates int and float. */
+enum ginf_dtype { GINF_F32 = 0, GINF_I32 = 1 };
+
+/* Smoothness models, same values as SmoothnessFunctions (src/include/grid.h:9-14). */
+enum ginf_sm_model { GINF_SMOOTH_EXPLICIT = 0, GINF_SMOOTH_POTTS = 1, GINF_SMOOTH_TRUNC_LINEAR = 2, GINF_SMOOTH_TRUNC_QUAD = 3 };
+
+/* Message normalisation: MEAN is the reference (src/bp.cc:44-54); MIN subtracts
+ * the message minimum (exact in FP32 for integer-valued costs). */
+enum ginf_norm { GINF_NORM_MEAN = 0, GINF_NORM_MIN = 1 };
+
+/* Half-sweep kernel family for the O(L) models (POTTS, TRUNC_LINEAR).
+ * CHUNKED: G lanes per node split the labels (exact for integer arithmetic).
+ * QUAD: four lanes per node, one per direction, each running the reference's
+ * sequential scans and normaliser (exact for every type and normalisation).
+ * AUTO: QUAD for FP32 mean normalisation with L > 16, else CHUNKED. */
+enum ginf_kernel { GINF_KERNEL_AUTO = 0, GINF_KERNEL_CHUNKED = 1, GINF_KERNEL_QUAD = 2 };
+
+typedef struct ginf_bp_desc {
+    int width;            /* W >= 1 */
+    int height;           /* H >= 1 */
+    int num_labels;       /* L >= 1 */
+    int dtype;            /* ginf_dtype */
+    int sm_model;         /* ginf_sm_model */
+    int num_levels;       /* >= 1; 1 = plain loopy BP (decodeBp) */
+    int iters_per_level;  /* >= 0 red-black half-sweeps per level */
+    int norm;             /* ginf_norm */
+    int lanes_per_node;   /* chunked kernel: 0 = default; else 1,2,4,8,16 lanes share a node's labels */
+    int kernel;           /* ginf_kernel */
+    int reserved[6];      /* must be zero */
+} ginf_bp_desc;
+
+typedef struct ginf_bp_plan_s *ginf_bp_plan;
+
+/* Allocate the device workspace for one problem shape.  sm_table_host is the
+ * L x L smoothness table in the reference layout V(a,b) at a + L*b
+ * (Matrix<T>(2,L,L), src/grid.cc:8), element type per desc->dtype. */
+int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, ginf_bp_plan *out);
+
+/* Run HBP on device-resident costs data_dev (L x H x W, x fastest, the layout
+ * of Grid<T>::getDataCosts()) and write interior labels of labels_dev
+ * (H x W int32, x fastest); border labels are left untouched, as the
+ * reference leaves them.  Asynchronous on `stream`. */
+int ginf_bp_plan_run(ginf_bp_plan plan, const void *data_dev, int32_t *labels_dev, void *stream);
+
+int ginf_bp_plan_destroy(ginf_bp_plan plan);
+
+/* Bytes of device workspace the plan holds. */
+int ginf_bp_plan_workspace_bytes(ginf_bp_plan plan, size_t *bytes);
+
+/* Synchronous host-pointer entry: the drop-in for decodeHbp/gpuDecodeHbp
+ * (decodeBp/gpuDecodeBp with num_levels = 1).  Copies costs to the device,
+ * runs, copies interior labels back into labels_host. */
+int ginf_hbp_host(const ginf_bp_desc *desc, const void *data_host, const void *sm_table_host,
+                  int32_t *labels_host);
+
+/* Labeling energy of Grid::getLabelingCost (src/grid.cc:102-125): interior
+ * nodes' D(f) + V(f, f_north) + V(f, f_east), summed in FP64 on the device in
+ * a fixed order.  table_dev is the L x L table on the device.  Synchronous:
+ * the result is written to *energy_host. */
+int ginf_energy_f64(int width, int height, int num_labels, int dtype, const void *data_dev,
+                    const void *table_dev, const int32_t *labels_dev, double *energy_host, void *stream);
+
+/* Test hook: copy the outbox messages of pyramid level `level` from the last
+ * run into msgs_host in the oracle layout [l][d][y][x] (Matrix<T>(4,w,h,4,L),
+ * src/bp.cc:125).  Synchronous. */
+int ginf_bp_plan_dump_msgs(ginf_bp_plan plan, int level, void *msgs_host);
+
+/* Profiling hook (no reference counterpart): while enabled, every steady-state
+ * half-sweep launch of ginf_bp_plan_run (all but the zero-input first
+ * half-sweep of the top level and the two labelling half-sweeps of level 0)
+ * is bracketed by a HIP event pair on the run's stream.  ginf_bp_plan_sweep_stats synchronises those events and returns the
+ * summed kernel time (ms), the number of timed launches and the node-label
+ * updates they performed (updated interior nodes x L); reset != 0 clears. */
+int ginf_bp_plan_set_sweep_timing(ginf_bp_plan plan, int enable);
+int ginf_bp_plan_sweep_stats(ginf_bp_plan plan, double *total_ms, int64_t *launches, double *node_label_updates,
+                             int reset);
+
+/* Node-label updates one ginf_bp_plan_run performs: the sum over levels and
+ * half-sweeps of (interior nodes of the moving colour) x L. */
+int ginf_bp_plan_updates_per_run(ginf_bp_plan plan, double *node_label_updates);
+
+/* Level dimensions of the plan's pyramid (ceil halving, src/bp.cc:101-102). */
+int ginf_bp_plan_level_dims(ginf_bp_plan plan, int level, int *w, int *h);
+
+/* Seeded synthetic stereo instance (project-owned generator, no reference
+ * counterpart): SplitMix64 stream `seed`; left = 3x3 box-smoothed uint8
+ * noise; disparity = background 2 plus `num_rects` rectangles with
+ * disparities in [0, dmax]; right(x-d, y) = left(x, y), nearer (larger d)
+ * winning, holes filled with noise; cost(l, y, x) = min(|L(x,y) - R(x-l,y)|,
+ * trunc), trunc where x - l < 0.  Any output pointer may be NULL.
+ * cost_f32 / cost_i32 are L x H x W. */
+int ginf_gen_stereo(uint64_t seed, int width, int height, int num_labels, int num_rects, int dmax,
+                    int trunc, uint8_t *left, uint8_t *right, int32_t *disp, float *cost_f32,
+                    int32_t *cost_i32);
+
+const char *ginf_last_error(void);
+
+/* Library version string. */
+const char *ginf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

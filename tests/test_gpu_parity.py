@@ -1,0 +1,177 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact labels and messages for integer arithmetic (T=int, or FP32
+with min normalisation on integer-valued costs) and for FP32 mean mode with
+one lane per node (same operation order as the reference); with several
+lanes per node FP32 mean-mode messages agree within 1e-5 relative (to the
+message scale), labels >= 99.99 % and the FP64 energy within 1e-6 relative.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+from gpu_util import run_plan, gpu_energy, say
+from gridinfcuda_amd import _lib as G
+
+pytestmark = pytest.mark.gpu
+
+SM = dict(explicit=0, potts=1, tl=2, tq=3)
+
+
+def table_for(o, kind, L, dtype, rng=None):
+    if kind == "potts":
+        return o.table_potts(L, 15, dtype)
+    if kind == "tl":
+        return o.table_trunc_linear(L, 8, 32, dtype)
+    if kind == "tq":
+        return o.table_trunc_quad(L, 3, 40, dtype)
+    return rng.integers(0, 30, size=(L, L)).astype(dtype)
+
+
+def stereo(seed, W, H, L, rects, dmax, trunc, dtype):
+    o = G.gen_stereo(seed, W, H, L, rects, dmax, trunc, want=("cost_f32", "cost_i32"))
+    return o["cost_i32"] if np.dtype(dtype) == np.int32 else o["cost_f32"]
+
+
+# ---------------------------------------------------------------- config 1 shape
+
+@pytest.mark.parametrize("dtype,norm", [(np.int32, 0), (np.float32, 1), (np.float32, 0), (np.int32, 1)])
+@pytest.mark.parametrize("lanes,kernel", [(0, 0), (1, 1), (2, 1), (4, 1), (0, 2)])
+def test_config1_bit_exact(oracle_mod, dtype, norm, lanes, kernel):
+    """128x96, 16 labels, TL (10,20), 10 half-sweeps, stereo seed 1."""
+    say(f"config1 dtype={np.dtype(dtype).name} norm={norm} lanes={lanes} kernel={kernel}")
+    data = stereo(1, 128, 96, 16, 8, 15, 20, dtype)
+    table = oracle_mod.table_trunc_linear(16, 10, 20, dtype)
+    ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, 2, 1, 10, norm=norm, return_state=True)
+    lab, msgs, _ = run_plan(data, table, 2, 1, 10, norm=norm, lanes=lanes, dump_level=0, runs=2, kernel=kernel)
+    exact = np.dtype(dtype) == np.int32 or norm == 1 or lanes in (0, 1) or kernel == 2
+    if exact:
+        np.testing.assert_array_equal(lab, ref_lab)
+        np.testing.assert_array_equal(msgs, ref_msgs)
+    else:
+        scale = max(1.0, float(np.abs(ref_msgs).max()))
+        assert np.abs(msgs - ref_msgs).max() <= 1e-5 * scale
+        assert (lab == ref_lab).mean() >= 0.9999
+    e_gpu = gpu_energy(data, table, lab)
+    assert e_gpu == oracle_mod.energy_f64(data, table, ref_lab) or not exact
+
+
+# ---------------------------------------------------------------- sweep of shapes
+
+def _cases():
+    out = []
+    rng = np.random.default_rng(1234)
+    shapes = [(1, 1), (2, 7), (3, 3), (3, 9), (5, 4), (7, 9), (16, 13), (33, 18), (40, 41)]
+    Ls = [2, 3, 5, 8, 16, 17, 31, 32, 33, 64, 65, 96, 100, 128, 129, 200, 256]
+    for k in range(60):
+        H, W = shapes[k % len(shapes)]
+        L = Ls[k % len(Ls)]
+        lanes = [0, 1, 2, 4, 8, 16][int(rng.integers(0, 6))]
+        levels = int(rng.integers(1, 5))
+        iters = int(rng.integers(0, 6))
+        kind = ["tl", "potts"][k % 2]
+        quad = k % 4 == 3
+        if quad:
+            dt, norm = [("float32", 0), ("int32", 0), ("float32", 1), ("int32", 1)][(k // 4) % 4]
+            out.append((H, W, L, 0, levels, iters, kind, dt, norm, 2))
+        else:
+            dt, norm = [("int32", 0), ("int32", 1), ("float32", 1)][k % 3]
+            out.append((H, W, L, lanes, levels, iters, kind, dt, norm, 1 if lanes else 0))
+    return out
+
+
+@pytest.mark.parametrize("H,W,L,lanes,levels,iters,kind,dt,norm,kernel", _cases())
+def test_linear_bit_exact(oracle_mod, H, W, L, lanes, levels, iters, kind, dt, norm, kernel):
+    say(f"linear H={H} W={W} L={L} lanes={lanes} levels={levels} iters={iters} {kind} {dt} norm={norm} "
+        f"kernel={kernel}")
+    rng = np.random.default_rng(zlib.crc32(repr((H, W, L, lanes, levels, iters, kind, dt, norm)).encode()))
+    dtype = np.dtype(dt)
+    data = rng.integers(0, 40, size=(L, H, W)).astype(dtype)
+    table = table_for(oracle_mod, kind, L, dtype)
+    init = rng.integers(0, L, size=(H, W)).astype(np.int32)
+    try:
+        lab, msgs, _ = run_plan(data, table, SM[kind], levels, iters, norm=norm, lanes=lanes, labels_init=init,
+                                dump_level=0, kernel=kernel)
+    except G.GinfError as e:
+        assert e.code == 2, e  # unsupported (lanes, L) pair
+        pytest.skip(str(e))
+    ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, SM[kind], levels, iters, norm=norm, labels_init=init,
+                                          return_state=True)
+    np.testing.assert_array_equal(lab, ref_lab)
+    np.testing.assert_array_equal(msgs, ref_msgs)
+
+
+@pytest.mark.parametrize("kind", ["tq", "explicit"])
+@pytest.mark.parametrize("dt", ["int32", "float32"])
+@pytest.mark.parametrize("L,levels,iters", [(5, 1, 4), (16, 2, 3), (33, 3, 5), (128, 2, 2)])
+def test_general_bit_exact(oracle_mod, kind, dt, L, levels, iters):
+    say(f"general {kind} {dt} L={L} levels={levels} iters={iters}")
+    rng = np.random.default_rng(L * 7 + iters)
+    dtype = np.dtype(dt)
+    H, W = 11, 14
+    data = rng.integers(0, 40, size=(L, H, W)).astype(dtype)
+    table = table_for(oracle_mod, kind, L, dtype, rng)
+    lab, msgs, _ = run_plan(data, table, SM[kind], levels, iters, dump_level=0)
+    ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, SM[kind], levels, iters, return_state=True)
+    np.testing.assert_array_equal(lab, ref_lab)
+    np.testing.assert_array_equal(msgs, ref_msgs)
+
+
+def test_float_mean_quad_exact_chunked_tolerance(oracle_mod):
+    """FP32 mean mode, 128 labels: the default (quad) kernel is bit-exact; the
+    chunked kernel, whose lanes split the labels, stays within 1e-5 relative."""
+    say("float mean 200x120x128")
+    data = stereo(5, 200, 120, 128, 16, 127, 40, np.float32)
+    table = oracle_mod.table_trunc_linear(128, 8, 32)
+    ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, 2, 3, 10, norm=0, return_state=True, threads=8)
+    lab, msgs, _ = run_plan(data, table, 2, 3, 10, norm=0, dump_level=0)
+    np.testing.assert_array_equal(lab, ref_lab)
+    np.testing.assert_array_equal(msgs, ref_msgs)
+    for lanes in (4, 16):
+        lab, msgs, _ = run_plan(data, table, 2, 3, 10, norm=0, lanes=lanes, dump_level=0, kernel=1)
+        scale = max(1.0, float(np.abs(ref_msgs).max()))
+        err = np.abs(msgs - ref_msgs).max() / scale
+        say(f"  lanes={lanes} max rel msg err {err:.3g} label agreement {(lab == ref_lab).mean():.6f}")
+        assert err <= 1e-5
+        assert (lab == ref_lab).mean() >= 0.9999
+        e_ref = oracle_mod.energy_f64(data, table, ref_lab)
+        assert abs(gpu_energy(data, table, lab) - e_ref) <= 1e-6 * abs(e_ref)
+
+
+def test_hbp_config2_reduced(oracle_mod):
+    """Config-2 recipe at reduced size: 480x270, 128 labels, TL lambda=8 tau=4,
+    5 levels x 10 half-sweeps, FP32 (reference mean normalisation)."""
+    say("config2 reduced 480x270x128")
+    data = stereo(2, 480, 270, 128, 64, 127, 40, np.float32)
+    table = oracle_mod.table_trunc_linear(128, 8, 32)
+    ref_lab = oracle_mod.hbp(data, table, 2, 5, 10, norm=0, threads=16)
+    lab, _, _ = run_plan(data, table, 2, 5, 10, norm=0)
+    agree = (lab == ref_lab).mean()
+    e_ref = oracle_mod.energy_f64(data, table, ref_lab)
+    e_gpu = gpu_energy(data, table, lab)
+    say(f"  agreement {agree:.6f} energy gpu {e_gpu} ref {e_ref}")
+    np.testing.assert_array_equal(lab, ref_lab)
+    assert e_gpu == e_ref
+
+
+def test_energy_kernel_matches_numpy(oracle_mod):
+    rng = np.random.default_rng(3)
+    for dt in ("int32", "float32"):
+        L, H, W = 40, 77, 123
+        data = rng.integers(0, 1000, size=(L, H, W)).astype(dt)
+        table = rng.integers(0, 50, size=(L, L)).astype(dt)
+        lab = rng.integers(0, L, size=(H, W)).astype(np.int32)
+        assert gpu_energy(data, table, lab) == oracle_mod.energy_f64(data, table, lab)
+
+
+def test_invalid_arguments_rejected():
+    with pytest.raises(G.GinfError) as e:
+        G.Plan(G.make_desc(10, 10, 4, num_levels=0), np.zeros((4, 4), np.float32))
+    assert e.value.code == 1
+    with pytest.raises(G.GinfError) as e:
+        G.Plan(G.make_desc(10, 10, 300), np.zeros((300, 300), np.float32))
+    assert e.value.code == 2
+    with pytest.raises(G.GinfError) as e:
+        G.Plan(G.make_desc(10, 10, 4, lanes_per_node=3), np.zeros((4, 4), np.float32))
+    assert e.value.code == 1
