@@ -9,16 +9,29 @@ namespace ginf {
 
 // Register-tile shape of a sweep.
 //   chunked (default): C labels per lane, G lanes per node split the labels
-//   quad:    four lanes per node, one per direction, C = label capacity; the
-//            state uses the one-lane-per-node layout (layout_g() == 1)
+//   quad:    four lanes per node, one per direction, C = label capacity
 //   general: the O(L^2) kernel, one lane per node, C >= L
 struct SweepShape {
     int C;
     int G;
     bool general;
     bool quad;
-    int layout_g() const { return quad ? 1 : G; }
 };
+
+// How a node's labels map onto the (plane i, lane q) grid of the state arrays
+// (element (i * nplane + slot) * G + q): label l = q * C + i, or with
+// `interleave` l = i * G + q (the quad kernel's 4-label groups).
+struct StateLayout {
+    int C;
+    int G;
+    bool interleave;
+};
+
+inline StateLayout state_layout(const SweepShape &sh, int L) {
+    if (sh.quad) return {(L + 3) / 4, 4, true};
+    if (sh.general) return {sh.C, 1, false};
+    return {sh.C, sh.G, false};
+}
 
 // Label capacities built for the quad kernel.
 #define GINF_QUAD_LIST(X) X(32) X(64) X(96) X(128) X(256)
@@ -56,7 +69,7 @@ hipError_t launch_sweep_quad(int dtype, const SweepShape &sh, const SweepArgs &a
                              hipStream_t st);
 hipError_t launch_sweep_general(int dtype, const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
                                 hipStream_t st);
-hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, int C, int G, const LevelGeom &g,
+hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, const StateLayout &ly, const LevelGeom &g,
                            void *dst0, void *dst1, hipStream_t st);
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
                           const LevelGeom &cg, int C, int G, hipStream_t st);

@@ -9,20 +9,21 @@ namespace ginf {
 namespace {
 
 // Reference-layout costs [l][y][x] -> level-0 device layout, both colours.
-// Labels >= L (padding up to C*G) get cost 0; they are masked in the sweeps.
+// Labels >= L (padding up to C*G) get cost 0; the sweeps mask them.
 template <typename T>
-__global__ __launch_bounds__(256) void k_relayout(const T *__restrict__ src, int W, int H, int L, int C, int G,
+__global__ __launch_bounds__(256) void k_relayout(const T *__restrict__ src, int W, int H, int L, StateLayout ly,
                                                   LevelGeom g, T *dst0, T *dst1) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= W) return;
     T *dst = ((x + y) & 1) ? dst1 : dst0;
     const int64_t slot = (int64_t)y * g.whp + (x >> 1);
-    const int LP = C * G;
+    const int LP = ly.C * ly.G;
     for (int l = 0; l < LP; ++l) {
-        const int qq = l / C, i = l - qq * C;
+        const int qq = ly.interleave ? l % ly.G : l / ly.C;
+        const int i = ly.interleave ? l / ly.G : l - qq * ly.C;
         const T v = (l < L) ? src[((int64_t)l * H + y) * W + x] : T(0);
-        dst[((int64_t)i * g.nplane + slot) * G + qq] = v;
+        dst[((int64_t)i * g.nplane + slot) * ly.G + qq] = v;
     }
 }
 
@@ -139,14 +140,14 @@ hipError_t launch_sweep(int dtype, const SweepShape &sh, const SweepArgs &a, boo
     return dtype == 0 ? launch_sweep_chunked_f32(sh, a, zeroin, labels, st)
                       : launch_sweep_chunked_i32(sh, a, zeroin, labels, st);
 }
-hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, int C, int G, const LevelGeom &g,
+hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, const StateLayout &ly, const LevelGeom &g,
                            void *dst0, void *dst1, hipStream_t st) {
     const dim3 grid((unsigned)((W + 255) / 256), (unsigned)H), block(256);
     if (dtype == 0)
-        hipLaunchKernelGGL(k_relayout<float>, grid, block, 0, st, (const float *)src, W, H, L, C, G, g,
-                           (float *)dst0, (float *)dst1);
+        hipLaunchKernelGGL(k_relayout<float>, grid, block, 0, st, (const float *)src, W, H, L, ly, g, (float *)dst0,
+                           (float *)dst1);
     else
-        hipLaunchKernelGGL(k_relayout<int>, grid, block, 0, st, (const int *)src, W, H, L, C, G, g, (int *)dst0,
+        hipLaunchKernelGGL(k_relayout<int>, grid, block, 0, st, (const int *)src, W, H, L, ly, g, (int *)dst0,
                            (int *)dst1);
     return hipGetLastError();
 }

@@ -48,6 +48,17 @@ struct SweepArgs {
     float s_f, tau_f;        // s = V(0,1), tau = V(0,L-1) (src/bp.cc:35,43)
     int s_i, tau_i;
     const void *table;       // L x L table (general model)
+    // Byte-addressed view used by the quad kernel (buffer loads with 32-bit
+    // offsets): the four outbox arrays of one colour are contiguous,
+    // arr_bytes apart, and colour 1's follow colour 0's.
+    const void *in_base;     // outbox array [colour 0 or the moving colour's partner][S] of the source level
+    void *out_base;          // outbox array [color][S] of this level
+    uint32_t arr_bytes;      // bytes between consecutive outbox arrays, this level
+    uint32_t in_arr_bytes;   // same, source level
+    uint32_t plane_bytes;    // bytes between register planes, this level
+    uint32_t in_plane_bytes; // same, source level
+    uint32_t data_bytes;     // bytes of the data-cost array of the moving colour
+    uint32_t in_span;        // bytes addressable from in_base
 };
 
 }  // namespace ginf

@@ -95,6 +95,7 @@ struct ginf_bp_plan_s {
     std::vector<Level> lv;
     void *pool = nullptr;
     size_t pool_bytes = 0;
+    std::vector<size_t> arr_bytes;  // per level: bytes between consecutive state arrays
     void *table_dev = nullptr;
     std::vector<unsigned char> table_host;
     float s_f = 0, tau_f = 0;
@@ -154,9 +155,10 @@ hipEvent_t take_event(ginf_bp_plan p) {
 int plan_run(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStream_t st) {
     using namespace ginf;
     const ginf_bp_desc &d = p->desc;
-    const int dt = d.dtype, C = p->shape.C, G = p->shape.layout_g(), T = p->levels, N = d.iters_per_level;
+    const StateLayout ly = state_layout(p->shape, d.num_labels);
+    const int dt = d.dtype, C = ly.C, G = ly.G, T = p->levels, N = d.iters_per_level;
     const int W = d.width, H = d.height, L = d.num_labels;
-    GINF_HIP_TRY(launch_relayout(dt, data_dev, W, H, L, C, G, p->geom[0], p->lv[0].data[0], p->lv[0].data[1], st));
+    GINF_HIP_TRY(launch_relayout(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0], p->lv[0].data[1], st));
     for (int t = 1; t < T; ++t)
         GINF_HIP_TRY(launch_pyramid(dt, p->lv[t - 1].data[0], p->lv[t - 1].data[1], p->geom[t - 1], p->lv[t].data[0],
                                     p->lv[t].data[1], p->geom[t], C, G, st));
@@ -210,6 +212,22 @@ int plan_run(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStrea
             a.label_w = W;
             a.s_f = p->s_f; a.tau_f = p->tau_f; a.s_i = p->s_i; a.tau_i = p->tau_i;
             a.table = p->table_dev;
+            // byte view for the quad kernel
+            a.arr_bytes = (uint32_t)p->arr_bytes[t];
+            a.data_bytes = (uint32_t)p->arr_bytes[t];
+            a.plane_bytes = (uint32_t)(g.nplane * G * 4);
+            a.out_base = p->lv[t].msgs[color][0];
+            if (src_level == t) {
+                a.in_base = p->lv[t].msgs[1 - color][0];
+                a.in_arr_bytes = a.arr_bytes;
+                a.in_plane_bytes = a.plane_bytes;
+                a.in_span = (uint32_t)(4 * p->arr_bytes[t]);
+            } else {
+                a.in_base = p->lv[src_level].msgs[0][0];
+                a.in_arr_bytes = (uint32_t)p->arr_bytes[src_level];
+                a.in_plane_bytes = (uint32_t)(p->geom[src_level].nplane * G * 4);
+                a.in_span = (uint32_t)(8 * p->arr_bytes[src_level]);
+            }
             const bool labels = (t == 0) && (i >= N - 2);
             ginf_bp_plan_s::Timed tm{nullptr, nullptr, 0.0};
             // Only the steady-state kernel instance (no zero-input, no labels) is
@@ -296,10 +314,23 @@ int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, gin
         g.nslots = g.whp * h;
         g.nplane = round_up(g.nslots, 64);
         p->geom.push_back(g);
-        const size_t arr = (size_t)sh.C * g.nplane * sh.layout_g() * es;
+        const ginf::StateLayout ly = ginf::state_layout(sh, L);
+        const size_t arr = (size_t)ly.C * g.nplane * ly.G * es;
         for (int k = 0; k < 10; ++k) { offs.push_back(total); total += round_up(arr, 256); }
+        p->arr_bytes.push_back(round_up(arr, 256));
     }
     total += round_up((size_t)L * L * es, 256);
+    if (sh.quad) {
+        // The quad kernel addresses a level's outbox arrays with 32-bit buffer
+        // offsets: 4 arrays of the level itself, 8 of the level above.
+        bool fits = 4 * (uint64_t)p->arr_bytes[0] < ((uint64_t)1 << 32);
+        for (int t = 1; t < p->levels; ++t) fits = fits && 8 * (uint64_t)p->arr_bytes[t] < ((uint64_t)1 << 32);
+        if (!fits) {
+            delete p;
+            return set_error(GINF_ERR_UNSUPPORTED, "grid too large for the quad kernel's 32-bit offsets "
+                                                   "(W*H*L must stay below about 2^29); use GINF_KERNEL_CHUNKED");
+        }
+    }
     p->pool_bytes = total;
     hipError_t e = hipMalloc(&p->pool, total);
     if (e != hipSuccess) {
@@ -398,7 +429,9 @@ int ginf_bp_plan_dump_msgs(ginf_bp_plan p, int level, void *msgs_host) {
     if (!p || !msgs_host) return set_error(GINF_ERR_INVALID, "NULL argument");
     if (level < 0 || level >= p->levels) return set_error(GINF_ERR_INVALID, "level %d out of range", level);
     const ginf::LevelGeom &g = p->geom[level];
-    const int C = p->shape.C, G = p->shape.layout_g(), L = p->desc.num_labels;
+    const int L = p->desc.num_labels;
+    const ginf::StateLayout ly = ginf::state_layout(p->shape, L);
+    const int C = ly.C, G = ly.G;
     const size_t n = (size_t)C * g.nplane * G;
     std::vector<uint32_t> buf(n);
     uint32_t *dst = (uint32_t *)msgs_host;
@@ -411,7 +444,8 @@ int ginf_bp_plan_dump_msgs(ginf_bp_plan p, int level, void *msgs_host) {
                     if (((x + y) & 1) != c) continue;
                     const size_t slot = (size_t)y * g.whp + (x >> 1);
                     for (int l = 0; l < L; ++l) {
-                        const int q = l / C, i = l - q * C;
+                        const int q = ly.interleave ? l % G : l / C;
+                        const int i = ly.interleave ? l / G : l - q * C;
                         dst[(size_t)x + (size_t)g.w * ((size_t)y + (size_t)g.h * ((size_t)d + 4u * l))] =
                             buf[((size_t)i * g.nplane + slot) * G + q];
                     }

@@ -24,8 +24,18 @@ namespace {
 constexpr int kDX[4] = {0, 1, 0, -1};
 constexpr int kDY[4] = {1, 0, -1, 0};
 
+// MIN of the reference, (a < b) ? a : b.  For float the hardware minimum
+// (v_min_f32) is used: it returns the same value for every pair of operands
+// except a NaN operand or a +0/-0 pair, which the reference inputs (finite
+// costs and tables without negative zeros) never produce, because no
+// operation of the algorithm yields -0 or NaN from such inputs.  Build with
+// -DGINF_STRICT_MIN to get the literal compare-and-select instead.
 template <typename T>
 __device__ __forceinline__ T tmin(T a, T b) { return (a < b) ? a : b; }
+#ifndef GINF_STRICT_MIN
+template <>
+__device__ __forceinline__ float tmin<float>(float a, float b) { return __builtin_fminf(a, b); }
+#endif
 
 template <typename T> struct Num;
 template <> struct Num<float> {

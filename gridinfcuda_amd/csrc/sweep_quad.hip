@@ -20,10 +20,15 @@ __device__ __forceinline__ int quad_bcast(int v) {
 // quad by DPP broadcasts) and then runs bpSendMsg for its own direction with
 // the reference's sequential scans and left-fold normaliser, so every
 // operation -- FP32 mean mode included -- rounds exactly as src/bp.cc:26-54.
-//   LMAX  register capacity in labels (L <= LMAX, checked at run time)
-// Layout: one lane per node (G = 1 planes).
-template <typename T, int LMAX, bool ZEROIN, bool LABELS>
+//   LMAX  register capacity in labels; FULL: L == LMAX (no label guards)
+// Layout: label quads, element (l, slot) at ((l/4) * nplane + slot) * 4 + l%4,
+// so one lane moves 4 labels (16 B) per access and 16 consecutive nodes of a
+// wave cover 256 contiguous bytes of every array.  Plane offsets are scalar
+// (buffer soffset); lane offsets are 32-bit.
+template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL>
 __global__ __launch_bounds__(256) void k_sweep_quad(const SweepArgs a) {
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    constexpr int LQ = LMAX / 4;
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int slot = gid >> 2;
     const int d = gid & 3;
@@ -31,59 +36,80 @@ __global__ __launch_bounds__(256) void k_sweep_quad(const SweepArgs a) {
     const int y = slot / a.whp;
     const int x = 2 * (slot - y * a.whp) + ((y + a.color) & 1);
     if (y < 1 || y > a.h - 2 || x < 1 || x > a.w - 2) return;  // whole quads leave together
-    const int L = a.num_labels;
+    const int L = FULL ? LMAX : a.num_labels;
     const T s = Num<T>::s(a);
     const T tau = Num<T>::tau(a);
-    const int64_t stride = a.stride;
-    const T *dp = (const T *)a.data + slot;
-    const T *ip = nullptr;
+
+    const __amdgpu_buffer_rsrc_t drs = __builtin_amdgcn_make_buffer_rsrc((void *)a.data, 0, a.data_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t irs = __builtin_amdgcn_make_buffer_rsrc((void *)a.in_base, 0, a.in_span, 0x00020000);
+    const __amdgpu_buffer_rsrc_t ors = __builtin_amdgcn_make_buffer_rsrc(a.out_base, 0, 4u * a.arr_bytes, 0x00020000);
+    const uint32_t self_off = (uint32_t)slot * 16u;
+    uint32_t in_off = 0;
     if (!ZEROIN) {
         int nx = x + kDX[d], ny = y + kDY[d];
-        if (a.src == kSrcCoarse) { nx >>= 1; ny >>= 1; }
+        int csel;
+        if (a.src == kSrcCoarse) { nx >>= 1; ny >>= 1; csel = (nx + ny) & 1; }
+        else csel = 0;  // in_base already points at the partner colour's arrays
         const int od = (d + 2) & 3;
-        const int c = (nx + ny) & 1;
-        ip = (const T *)a.in[c][od] + ((int64_t)ny * a.in_whp + (nx >> 1));
+        in_off = (uint32_t)(csel * 4 + od) * a.in_arr_bytes + (uint32_t)(ny * a.in_whp + (nx >> 1)) * 16u;
     }
+    const uint32_t out_off = (uint32_t)d * a.arr_bytes + self_off;
 
     T f[LMAX];
     T mu = T(0), best = T(0);
     int arg = 0;
 #pragma unroll
-    for (int i = 0; i < LMAX; ++i) {
-        if (i < L) {
-            const T m = ZEROIN ? T(0) : ip[i * a.in_stride];
-            T b = dp[i * stride];
-            b = b + quad_bcast<0>(m);
-            b = b + quad_bcast<1>(m);
-            b = b + quad_bcast<2>(m);
-            b = b + quad_bcast<3>(m);
-            if (LABELS) {
-                if (i == 0) { best = b; arg = 0; }
-                else if (b < best) { best = b; arg = i; }
+    for (int j = 0; j < LQ; ++j) {
+        if (FULL || 4 * j < L) {
+            v4 m4 = v4(T(0));
+            if (!ZEROIN)
+                m4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(irs, in_off, j * a.in_plane_bytes, 0));
+            const v4 d4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(drs, self_off, j * a.plane_bytes, 0));
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int l = 4 * j + r;
+                if (FULL || l < L) {
+                    const T m = m4[r];
+                    T b = d4[r];
+                    b = b + quad_bcast<0>(m);
+                    b = b + quad_bcast<1>(m);
+                    b = b + quad_bcast<2>(m);
+                    b = b + quad_bcast<3>(m);
+                    if (LABELS) {
+                        if (l == 0) { best = b; arg = 0; }
+                        else if (b < best) { best = b; arg = l; }
+                    }
+                    const T h = b - m;
+                    if (l == 0) { f[0] = h; mu = h; }
+                    else { f[l] = tmin(h, f[l - 1] + s); mu = tmin(mu, h); }
+                }
             }
-            const T h = b - m;
-            if (i == 0) { f[0] = h; mu = h; }
-            else { f[i] = tmin(h, f[i - 1] + s); mu = tmin(mu, h); }
         }
     }
 #pragma unroll
-    for (int i = LMAX - 2; i >= 0; --i)
-        if (i <= L - 2) f[i] = tmin(f[i], f[i + 1] + s);
+    for (int l = LMAX - 2; l >= 0; --l)
+        if (FULL || l <= L - 2) f[l] = tmin(f[l], f[l + 1] + s);
     const T cap = mu + tau;
     T acc = T(0), mn = T(0);
 #pragma unroll
-    for (int i = 0; i < LMAX; ++i) {
-        if (i < L) {
-            f[i] = tmin(f[i], cap);
-            acc = acc + f[i];
-            mn = (i == 0) ? f[0] : tmin(mn, f[i]);
+    for (int l = 0; l < LMAX; ++l) {
+        if (FULL || l < L) {
+            f[l] = tmin(f[l], cap);
+            acc = acc + f[l];
+            mn = (l == 0) ? f[0] : tmin(mn, f[l]);
         }
     }
     const T nu = (a.norm == 0) ? acc / T(L) : mn;
-    T *op = (T *)a.out[d] + slot;
 #pragma unroll
-    for (int i = 0; i < LMAX; ++i)
-        if (i < L) op[i * stride] = f[i] - nu;
+    for (int j = 0; j < LQ; ++j) {
+        if (FULL || 4 * j < L) {
+            v4 o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) o[r] = (FULL || 4 * j + r < L) ? f[4 * j + r] - nu : T(0);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, o),
+                                                   ors, out_off, j * a.plane_bytes, 0);
+        }
+    }
     if (LABELS && d == 0) a.labels[(int64_t)y * a.label_w + x] = arg;
 }
 
@@ -91,10 +117,16 @@ template <typename T, int LMAX>
 hipError_t launch_quad_l(const SweepArgs &a, bool zeroin, bool labels, hipStream_t st) {
     const int64_t threads = (int64_t)a.nslots * 4;
     const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
-    if (zeroin) { if (labels) hipLaunchKernelGGL((k_sweep_quad<T, LMAX, true, true>), grid, block, 0, st, a);
-                  else hipLaunchKernelGGL((k_sweep_quad<T, LMAX, true, false>), grid, block, 0, st, a); }
-    else { if (labels) hipLaunchKernelGGL((k_sweep_quad<T, LMAX, false, true>), grid, block, 0, st, a);
-           else hipLaunchKernelGGL((k_sweep_quad<T, LMAX, false, false>), grid, block, 0, st, a); }
+    const bool full = a.num_labels == LMAX;
+#define GINF_QK(Z, LB, F) hipLaunchKernelGGL((k_sweep_quad<T, LMAX, Z, LB, F>), grid, block, 0, st, a)
+    if (full) {
+        if (zeroin) { if (labels) GINF_QK(true, true, true); else GINF_QK(true, false, true); }
+        else { if (labels) GINF_QK(false, true, true); else GINF_QK(false, false, true); }
+    } else {
+        if (zeroin) { if (labels) GINF_QK(true, true, false); else GINF_QK(true, false, false); }
+        else { if (labels) GINF_QK(false, true, false); else GINF_QK(false, false, false); }
+    }
+#undef GINF_QK
     return hipGetLastError();
 }
 
