@@ -90,6 +90,8 @@ def main() -> int:
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 chunked, 2 quad (per-direction lanes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (profiler counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -116,6 +118,13 @@ def main() -> int:
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
 
+    if args.copy_mib:
+        n = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         plan.run(data.data_ptr(), labels.data_ptr(), sp)
     torch.cuda.synchronize()

@@ -71,6 +71,9 @@ hipError_t launch_sweep_general(int dtype, const SweepShape &sh, const SweepArgs
                                 hipStream_t st);
 hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, const StateLayout &ly, const LevelGeom &g,
                            void *dst0, void *dst1, hipStream_t st);
+hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
+                               const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
+                               void *d11, hipStream_t st);
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
                           const LevelGeom &cg, int C, int G, hipStream_t st);
 hipError_t launch_upsample(int dtype, const UpsampleArgs &u, hipStream_t st);

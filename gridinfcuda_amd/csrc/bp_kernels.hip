@@ -27,6 +27,91 @@ __global__ __launch_bounds__(256) void k_relayout(const T *__restrict__ src, int
     }
 }
 
+// Relayout fused with the first pyramid level.  A block takes fine rows
+// 2Y and 2Y+1 and TW consecutive columns of the reference-layout costs
+// [l][y][x], stages them in LDS (coalesced reads along x), then writes the
+// level-0 state (both colours; each label plane's slots are contiguous) and,
+// when `coarse` is set, the level-1 costs of coarse row Y: the children
+// (2X,2Y), (2X+1,2Y), (2X,2Y+1), (2X+1,2Y+1) added in that order from 0
+// (src/bp.cc:100-117).  Padding labels (L <= l < C*G) get cost 0.
+template <typename T, int TW>
+__global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src, int W, int H, int L, StateLayout ly,
+                                                      LevelGeom g0, T *d00, T *d01, int coarse, LevelGeom g1, T *d10,
+                                                      T *d11) {
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int TP = TW + 1;  // padded row: label-strided LDS reads hit distinct banks
+    T *tile = reinterpret_cast<T *>(smem_raw);  // [2][LP][TP]
+    const int G = ly.G, C = ly.C;
+    const int gs = __builtin_ctz(G);  // G is a power of two
+    const int LP = C * G;
+    const int x0 = blockIdx.x * TW;
+    const int Y = blockIdx.y;
+    const int tid = threadIdx.x;
+    // stage: 2 rows x LP labels x TW columns, columns fastest
+    for (int rl = tid / TW; rl < 2 * LP; rl += 256 / TW) {
+        const int xl = tid % TW;
+        const int rr = rl >= LP ? 1 : 0;
+        const int l = rl - rr * LP;
+        const int x = x0 + xl, y = 2 * Y + rr;
+        T v = T(0);
+        if (l < L && x < W && y < H) v = src[((int64_t)l * H + y) * W + x];
+        tile[rl * TP + xl] = v;
+    }
+    __syncthreads();
+    // level 0: per (row, colour, plane i): HS slots x G lanes, contiguous in memory
+    constexpr int HS = TW / 2;
+    {
+        const int per = HS << gs;          // elements per (row, colour, plane)
+        const int ni = 256 / per > 0 ? 256 / per : 1;
+        const int sub = tid / per, r = tid % per;
+        const int q = r & (G - 1), k = r >> gs;
+        if (sub < ni) {
+            for (int rc = 0; rc < 4; ++rc) {
+                const int rr = rc >> 1, c = rc & 1;
+                const int y = 2 * Y + rr;
+                const int xl = 2 * k + ((y + c) & 1);
+                const int x = x0 + xl;
+                if (y >= H || x >= W) continue;
+                const int64_t slot = (int64_t)y * g0.whp + (x >> 1);
+                T *dst = c ? d01 : d00;
+                for (int i = sub; i < C; i += ni) {
+                    const int l = ly.interleave ? (i << gs) + q : q * C + i;
+                    dst[((int64_t)i * g0.nplane + slot) * G + q] = tile[(rr * LP + l) * TP + xl];
+                }
+            }
+        }
+    }
+    if (!coarse) return;
+    // level 1: coarse row Y, QS slots per colour
+    constexpr int QS = TW / 4;
+    {
+        const int per = QS << gs;
+        const int ni = 256 / per > 0 ? 256 / per : 1;
+        const int sub = tid / per, r = tid % per;
+        const int q = r & (G - 1), k = r >> gs;
+        if (sub >= ni) return;
+        for (int c = 0; c < 2; ++c) {
+            const int Xl = 2 * k + ((Y + c) & 1);
+            const int X = x0 / 2 + Xl;
+            if (X >= g1.w) continue;
+            const bool ex = 2 * X + 1 < W, ey = 2 * Y + 1 < H;
+            const int64_t slot = (int64_t)Y * g1.whp + (X >> 1);
+            T *dst = c ? d11 : d10;
+            for (int i = sub; i < C; i += ni) {
+                const int l = ly.interleave ? (i << gs) + q : q * C + i;
+                const T *t0 = tile + l * TP;
+                const T *t1 = tile + (LP + l) * TP;
+                T acc = T(0);
+                acc = acc + t0[2 * Xl];
+                if (ex) acc = acc + t0[2 * Xl + 1];
+                if (ey) acc = acc + t1[2 * Xl];
+                if (ex && ey) acc = acc + t1[2 * Xl + 1];
+                dst[((int64_t)i * g1.nplane + slot) * G + q] = acc;
+            }
+        }
+    }
+}
+
 // Data-cost pyramid level (src/bp.cc:100-117): parent (X,Y) sums its children
 // (2X,2Y), (2X+1,2Y), (2X,2Y+1), (2X+1,2Y+1) in that order, starting from 0.
 // Children (2X,2Y) and (2X+1,2Y+1) have colour 0, the other two colour 1, and
@@ -149,6 +234,23 @@ hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, cons
     else
         hipLaunchKernelGGL(k_relayout<int>, grid, block, 0, st, (const int *)src, W, H, L, ly, g, (int *)dst0,
                            (int *)dst1);
+    return hipGetLastError();
+}
+
+hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
+                               const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
+                               void *d11, hipStream_t st) {
+    const int LP = ly.C * ly.G;
+    const int tw = LP > 64 ? 32 : 64;
+    const size_t lds = (size_t)2 * LP * (tw + 1) * 4;
+    if (lds > 160 * 1024 || (tw / 2) * ly.G > 256) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((W + tw - 1) / tw), (unsigned)((H + 1) / 2)), block(256);
+#define GINF_RP(TT, TWW)                                                                                            \
+    hipLaunchKernelGGL((k_relayout_pyr<TT, TWW>), grid, block, lds, st, (const TT *)src, W, H, L, ly, g0, (TT *)d00, \
+                       (TT *)d01, coarse ? 1 : 0, g1, (TT *)d10, (TT *)d11)
+    if (dtype == 0) { if (tw == 32) GINF_RP(float, 32); else GINF_RP(float, 64); }
+    else { if (tw == 32) GINF_RP(int, 32); else GINF_RP(int, 64); }
+#undef GINF_RP
     return hipGetLastError();
 }
 

@@ -158,8 +158,11 @@ int plan_run(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStrea
     const StateLayout ly = state_layout(p->shape, d.num_labels);
     const int dt = d.dtype, C = ly.C, G = ly.G, T = p->levels, N = d.iters_per_level;
     const int W = d.width, H = d.height, L = d.num_labels;
-    GINF_HIP_TRY(launch_relayout(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0], p->lv[0].data[1], st));
-    for (int t = 1; t < T; ++t)
+    // Level 0 (and level 1) costs in one pass over the caller's array.
+    const bool two = T > 1;
+    GINF_HIP_TRY(launch_relayout_pyr(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0], p->lv[0].data[1], two,
+                                     p->geom[two ? 1 : 0], p->lv[two ? 1 : 0].data[0], p->lv[two ? 1 : 0].data[1], st));
+    for (int t = 2; t < T; ++t)
         GINF_HIP_TRY(launch_pyramid(dt, p->lv[t - 1].data[0], p->lv[t - 1].data[1], p->geom[t - 1], p->lv[t].data[0],
                                     p->lv[t].data[1], p->geom[t], C, G, st));
     if (N == 0) {

@@ -56,8 +56,12 @@ __global__ __launch_bounds__(256) void k_sweep_quad(const SweepArgs a) {
     const uint32_t out_off = (uint32_t)d * a.arr_bytes + self_off;
 
     T f[LMAX];
-    T mu = T(0), best = T(0);
-    int arg = 0;
+    T mu = T(0);
+    // Labels: lane d tracks the belief minimum over labels l = 4j + d only
+    // (strict <, so the lowest such label wins); the quad combines the four
+    // residues at the end.
+    T best = T(0);
+    int arg = -1;
 #pragma unroll
     for (int j = 0; j < LQ; ++j) {
         if (FULL || 4 * j < L) {
@@ -65,24 +69,34 @@ __global__ __launch_bounds__(256) void k_sweep_quad(const SweepArgs a) {
             if (!ZEROIN)
                 m4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(irs, in_off, j * a.in_plane_bytes, 0));
             const v4 d4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(drs, self_off, j * a.plane_bytes, 0));
+            // Beliefs of the four labels, D + in_S + in_E + in_N + in_W added
+            // left to right (a + b == b + a exactly, so the broadcast operand
+            // may come first).  The four chains are interleaved so that no DPP
+            // read waits on the add just before it.
+            v4 b4;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b4[r] = quad_bcast<0>(m4[r]) + d4[r];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b4[r] = quad_bcast<1>(m4[r]) + b4[r];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b4[r] = quad_bcast<2>(m4[r]) + b4[r];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b4[r] = quad_bcast<3>(m4[r]) + b4[r];
+            T bsel = T(0);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int l = 4 * j + r;
                 if (FULL || l < L) {
                     const T m = m4[r];
-                    T b = d4[r];
-                    b = b + quad_bcast<0>(m);
-                    b = b + quad_bcast<1>(m);
-                    b = b + quad_bcast<2>(m);
-                    b = b + quad_bcast<3>(m);
-                    if (LABELS) {
-                        if (l == 0) { best = b; arg = 0; }
-                        else if (b < best) { best = b; arg = l; }
-                    }
+                    const T b = b4[r];
+                    if (LABELS && r == d) bsel = b;
                     const T h = b - m;
                     if (l == 0) { f[0] = h; mu = h; }
                     else { f[l] = tmin(h, f[l - 1] + s); mu = tmin(mu, h); }
                 }
+            }
+            if (LABELS && (FULL || 4 * j + d < L)) {
+                if (arg < 0 || bsel < best) { best = bsel; arg = 4 * j + d; }
             }
         }
     }
@@ -110,7 +124,17 @@ __global__ __launch_bounds__(256) void k_sweep_quad(const SweepArgs a) {
                                                    ors, out_off, j * a.plane_bytes, 0);
         }
     }
-    if (LABELS && d == 0) a.labels[(int64_t)y * a.label_w + x] = arg;
+    if (LABELS) {
+        // Combine the residues: smaller belief wins, ties go to the lower label.
+#pragma unroll
+        for (int k = 1; k < 4; k <<= 1) {
+            const T ob = __shfl_xor(best, k, 4);
+            const int oa = __shfl_xor(arg, k, 4);
+            const bool take = oa >= 0 && (arg < 0 || ob < best || (!(best < ob) && oa < arg));
+            if (take) { best = ob; arg = oa; }
+        }
+        if (d == 0) a.labels[(int64_t)y * a.label_w + x] = arg;
+    }
 }
 
 template <typename T, int LMAX>
