@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libginf_bp.so")
+# GINF_LIB selects another build of the same library (e.g. an experimental variant).
+LIB_PATH = os.environ.get("GINF_LIB") or os.path.join(PKG, "libginf_bp.so")
 
 F32, I32 = 0, 1
 SM_EXPLICIT, SM_POTTS, SM_TRUNC_LINEAR, SM_TRUNC_QUAD = 0, 1, 2, 3
@@ -23,7 +24,7 @@ EXPORTS = [
     "ginf_bp_plan_create", "ginf_bp_plan_run", "ginf_bp_plan_destroy", "ginf_bp_plan_workspace_bytes",
     "ginf_hbp_host", "ginf_energy_f64", "ginf_bp_plan_dump_msgs", "ginf_bp_plan_level_dims",
     "ginf_gen_stereo", "ginf_last_error", "ginf_version", "ginf_bp_plan_set_sweep_timing",
-    "ginf_bp_plan_sweep_stats", "ginf_bp_plan_updates_per_run",
+    "ginf_bp_plan_sweep_stats", "ginf_bp_plan_updates_per_run", "ginf_bp_plan_dump_data",
 ]
 
 
@@ -68,6 +69,7 @@ def lib() -> ctypes.CDLL:
         L.ginf_hbp_host.argtypes = [P(BpDesc), VP, VP, VP]
         L.ginf_energy_f64.argtypes = [I, I, I, I, VP, VP, VP, P(ctypes.c_double), VP]
         L.ginf_bp_plan_dump_msgs.argtypes = [VP, I, VP]
+        L.ginf_bp_plan_dump_data.argtypes = [VP, I, VP]
         L.ginf_bp_plan_level_dims.argtypes = [VP, I, P(I), P(I)]
         L.ginf_gen_stereo.argtypes = [ctypes.c_uint64, I, I, I, I, I, I, VP, VP, VP, VP, VP]
         L.ginf_bp_plan_set_sweep_timing.argtypes = [VP, I]
@@ -133,6 +135,14 @@ class Plan:
         L = self.desc.num_labels
         out = np.zeros((L, 4, h, w), np.float32 if self.desc.dtype == F32 else np.int32)
         check(lib().ginf_bp_plan_dump_msgs(self._h, level, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def dump_data(self, level: int = 0) -> np.ndarray:
+        """Data costs of a pyramid level as (L, h, w)."""
+        w, h = self.level_dims(level)
+        L = self.desc.num_labels
+        out = np.zeros((L, h, w), np.float32 if self.desc.dtype == F32 else np.int32)
+        check(lib().ginf_bp_plan_dump_data(self._h, level, out.ctypes.data_as(ctypes.c_void_p)))
         return out
 
     def set_sweep_timing(self, enable: bool) -> None:

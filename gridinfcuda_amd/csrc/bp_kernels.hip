@@ -47,15 +47,28 @@ __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src,
     const int x0 = blockIdx.x * TW;
     const int Y = blockIdx.y;
     const int tid = threadIdx.x;
-    // stage: 2 rows x LP labels x TW columns, columns fastest
-    for (int rl = tid / TW; rl < 2 * LP; rl += 256 / TW) {
+    // stage: 2 rows x LP labels x TW columns, columns fastest; eight loads
+    // in flight per thread before their LDS writes
+    {
+        constexpr int STEP = 256 / TW;
         const int xl = tid % TW;
-        const int rr = rl >= LP ? 1 : 0;
-        const int l = rl - rr * LP;
-        const int x = x0 + xl, y = 2 * Y + rr;
-        T v = T(0);
-        if (l < L && x < W && y < H) v = src[((int64_t)l * H + y) * W + x];
-        tile[rl * TP + xl] = v;
+        const int x = x0 + xl;
+        for (int base = tid / TW; base < 2 * LP; base += 8 * STEP) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int rl = base + u * STEP;
+                const int rr = rl >= LP ? 1 : 0;
+                const int l = rl - rr * LP;
+                const int y = 2 * Y + rr;
+                v[u] = (rl < 2 * LP && l < L && x < W && y < H) ? src[((int64_t)l * H + y) * W + x] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int rl = base + u * STEP;
+                if (rl < 2 * LP) tile[rl * TP + xl] = v[u];
+            }
+        }
     }
     __syncthreads();
     // level 0: per (row, colour, plane i): HS slots x G lanes, contiguous in memory

@@ -174,7 +174,11 @@ int plan_run(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStrea
     // With two or more half-sweeps per level both colours move, so the first
     // half-sweep of a level can read the level above directly (fused
     // upsample), and the top level's first half-sweep reads zero messages.
+#ifdef GINF_DEBUG_NO_FUSE
+    const bool fused = false;
+#else
     const bool fused = N >= 2;
+#endif
     for (int t = T - 1; t >= 0; --t) {
         const LevelGeom &g = p->geom[t];
         const int64_t per = (int64_t)C * g.nplane * G;
@@ -242,6 +246,9 @@ int plan_run(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStrea
                 if (tm.beg) GINF_HIP_TRY(hipEventRecord(tm.beg, st));
             }
             GINF_HIP_TRY(launch_sweep(dt, p->shape, a, zeroin, labels, st));
+#ifdef GINF_DEBUG_SYNC
+            GINF_HIP_TRY(hipDeviceSynchronize());
+#endif
             if (tm.beg && tm.end) {
                 GINF_HIP_TRY(hipEventRecord(tm.end, st));
                 p->timed.push_back(tm);
@@ -454,6 +461,34 @@ int ginf_bp_plan_dump_msgs(ginf_bp_plan p, int level, void *msgs_host) {
                     }
                 }
         }
+    return GINF_OK;
+}
+
+int ginf_bp_plan_dump_data(ginf_bp_plan p, int level, void *data_host) {
+    if (!p || !data_host) return set_error(GINF_ERR_INVALID, "NULL argument");
+    if (level < 0 || level >= p->levels) return set_error(GINF_ERR_INVALID, "level %d out of range", level);
+    const ginf::LevelGeom &g = p->geom[level];
+    const int L = p->desc.num_labels;
+    const ginf::StateLayout ly = ginf::state_layout(p->shape, L);
+    const int C = ly.C, G = ly.G;
+    const size_t n = (size_t)C * g.nplane * G;
+    std::vector<uint32_t> buf(n);
+    uint32_t *dst = (uint32_t *)data_host;
+    GINF_HIP_TRY(hipDeviceSynchronize());
+    for (int c = 0; c < 2; ++c) {
+        GINF_HIP_TRY(hipMemcpy(buf.data(), p->lv[level].data[c], n * 4, hipMemcpyDeviceToHost));
+        for (int y = 0; y < g.h; ++y)
+            for (int x = 0; x < g.w; ++x) {
+                if (((x + y) & 1) != c) continue;
+                const size_t slot = (size_t)y * g.whp + (x >> 1);
+                for (int l = 0; l < L; ++l) {
+                    const int q = ly.interleave ? l % G : l / C;
+                    const int i = ly.interleave ? l / G : l - q * C;
+                    dst[(size_t)x + (size_t)g.w * ((size_t)y + (size_t)g.h * (size_t)l)] =
+                        buf[((size_t)i * g.nplane + slot) * G + q];
+                }
+            }
+    }
     return GINF_OK;
 }
 

@@ -1,6 +1,16 @@
 // Quad-per-node O(L) half-sweep kernels (exact reference operation order).
 #include "sweep_common.h"
 
+// Cache policy of the streaming buffer accesses (0 = default; 2 = non-temporal).
+// Outboxes are not read again within the half-sweep, so they are stored
+// non-temporally (measured faster on 1920x1080x128); loads keep the default.
+#ifndef GINF_QUAD_LOAD_POLICY
+#define GINF_QUAD_LOAD_POLICY 0
+#endif
+#ifndef GINF_QUAD_STORE_POLICY
+#define GINF_QUAD_STORE_POLICY 2
+#endif
+
 namespace ginf {
 namespace {
 
@@ -23,8 +33,11 @@ __device__ __forceinline__ int quad_bcast(int v) {
 //   LMAX  register capacity in labels; FULL: L == LMAX (no label guards)
 // Layout: label quads, element (l, slot) at ((l/4) * nplane + slot) * 4 + l%4,
 // so one lane moves 4 labels (16 B) per access and 16 consecutive nodes of a
-// wave cover 256 contiguous bytes of every array.  Plane offsets are scalar
-// (buffer soffset); lane offsets are 32-bit.
+// wave cover 256 contiguous bytes of every array.  Offsets are 32-bit and
+// carried entirely in the VGPR offset (soffset = 0): with the plane offset in
+// soffset, the compiler reused SGPRs that a v_cmp in the labelling code had
+// just written, and the memory instruction then intermittently read the wrong
+// offset (nondeterministic message corruption, measured on gfx950).
 template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL>
 __global__ __launch_bounds__(256) void k_sweep_quad(const SweepArgs a) {
     typedef T v4 __attribute__((ext_vector_type(4)));
@@ -67,8 +80,8 @@ __global__ __launch_bounds__(256) void k_sweep_quad(const SweepArgs a) {
         if (FULL || 4 * j < L) {
             v4 m4 = v4(T(0));
             if (!ZEROIN)
-                m4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(irs, in_off, j * a.in_plane_bytes, 0));
-            const v4 d4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(drs, self_off, j * a.plane_bytes, 0));
+                m4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(irs, in_off + (uint32_t)j * a.in_plane_bytes, 0, GINF_QUAD_LOAD_POLICY));
+            const v4 d4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(drs, self_off + (uint32_t)j * a.plane_bytes, 0, GINF_QUAD_LOAD_POLICY));
             // Beliefs of the four labels, D + in_S + in_E + in_N + in_W added
             // left to right (a + b == b + a exactly, so the broadcast operand
             // may come first).  The four chains are interleaved so that no DPP
@@ -95,7 +108,11 @@ __global__ __launch_bounds__(256) void k_sweep_quad(const SweepArgs a) {
                     else { f[l] = tmin(h, f[l - 1] + s); mu = tmin(mu, h); }
                 }
             }
+#ifdef GINF_DEBUG_NO_ARGMIN
+            if (false) {
+#else
             if (LABELS && (FULL || 4 * j + d < L)) {
+#endif
                 if (arg < 0 || bsel < best) { best = bsel; arg = 4 * j + d; }
             }
         }
@@ -121,7 +138,7 @@ __global__ __launch_bounds__(256) void k_sweep_quad(const SweepArgs a) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) o[r] = (FULL || 4 * j + r < L) ? f[4 * j + r] - nu : T(0);
             __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, o),
-                                                   ors, out_off, j * a.plane_bytes, 0);
+                                                   ors, out_off + (uint32_t)j * a.plane_bytes, 0, GINF_QUAD_STORE_POLICY);
         }
     }
     if (LABELS) {

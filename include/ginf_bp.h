@@ -115,6 +115,11 @@ int ginf_bp_plan_sweep_stats(ginf_bp_plan plan, double *total_ms, int64_t *launc
  * half-sweeps of (interior nodes of the moving colour) x L. */
 int ginf_bp_plan_updates_per_run(ginf_bp_plan plan, double *node_label_updates);
 
+/* Test hook: copy the data costs of pyramid level `level` from the last run
+ * into data_host in the reference layout [l][y][x] (Matrix<T>(3,w,h,L),
+ * src/bp.cc:96,109).  Synchronous. */
+int ginf_bp_plan_dump_data(ginf_bp_plan plan, int level, void *data_host);
+
 /* Level dimensions of the plan's pyramid (ceil halving, src/bp.cc:101-102). */
 int ginf_bp_plan_level_dims(ginf_bp_plan plan, int level, int *w, int *h);
 

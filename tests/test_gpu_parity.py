@@ -175,3 +175,18 @@ def test_invalid_arguments_rejected():
     with pytest.raises(G.GinfError) as e:
         G.Plan(G.make_desc(10, 10, 4, lanes_per_node=3), np.zeros((4, 4), np.float32))
     assert e.value.code == 1
+
+
+@pytest.mark.parametrize("iters", [2, 3])
+def test_labelling_sweeps_repeatable(oracle_mod, iters):
+    """Regression: the labelling half-sweeps at a size with many waves in
+    flight give the oracle's messages on every repetition (an earlier build
+    corrupted them intermittently through a reused scalar offset)."""
+    say(f"labelling sweeps 480x270x128 iters={iters}")
+    data = stereo(2, 480, 270, 128, 64, 127, 40, np.float32)
+    table = oracle_mod.table_trunc_linear(128, 8, 32)
+    ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, 2, 1, iters, norm=0, return_state=True, threads=16)
+    for rep in range(3):
+        lab, msgs, _ = run_plan(data, table, 2, 1, iters, norm=0, dump_level=0)
+        assert int((msgs != ref_msgs).sum()) == 0, f"rep {rep}"
+        np.testing.assert_array_equal(lab, ref_lab)
