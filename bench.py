@@ -43,11 +43,11 @@ def cpu_model() -> str:
 
 def cpu_baseline(threads_all: int) -> dict:
     """The oracle (plain-C restatement of the reference CPU BP) on a bounded
-    sample of the same workload: the config recipe at 480x270 (1/16 of the
+    sample of the same workload: the config recipe at 960x540 (1/4 of the
     pixels), 128 labels, 5 levels x 10 half-sweeps."""
     from oracle import oracle as O
     from gridinfcuda_amd import _lib as G
-    W, H, L = 480, 270, 128
+    W, H, L = 960, 540, 128
     data = G.gen_stereo(2, W, H, L, 64, 127, 40)["cost_f32"]
     table = O.table_trunc_linear(L, 8, 32)
     # node-label updates of the sample: same counting as the GPU metric
@@ -69,7 +69,7 @@ def cpu_baseline(threads_all: int) -> dict:
         res[th] = plan_u / dt / 1e9
     return {
         "value": res[1], "unit": "Gnode-label msg updates/s", "cores": 1, "kind": "port",
-        "sample": f"480x270x128 HBP 5 levels x 10 half-sweeps (config recipe at 1/16 of the pixels), "
+        "sample": f"960x540x128 HBP 5 levels x 10 half-sweeps (config recipe at 1/4 of the pixels), "
                   f"{plan_u:.4g} node-label updates, oracle built -O2 -m64",
         "value_all_cores": res[threads_all], "cores_all": threads_all, "cpu_model": cpu_model(),
     }
@@ -148,6 +148,15 @@ def main() -> int:
     bytes_per_launch = BYTES_PER_UPDATE * sweep_updates / max(1, launches)
     achieved = bytes_per_launch / (avg_launch_ms / 1000.0) / 1e9
 
+    # HBM bytes per steady half-sweep launch, measured with rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE (scaled by a 1 GiB copy) on this workload; see the profile summary.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "sweep_quad_1920x1080x128_traffic.json")
+    default_shape = (W, H, L, args.levels, args.iters, args.norm, args.kernel, args.lanes) == (1920, 1080, 128, 5, 10, 0, 0, 0)
+    if default_shape and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
+
     # FP64 energy of the result (outside the timed region).
     tdev = torch.from_numpy(table).cuda()
     energy = G.energy_f64(W, H, L, G.F32, data.data_ptr(), tdev.data_ptr(), labels.data_ptr(), sp)
@@ -179,8 +188,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "k_sweep_linear steady-state instance (half-sweep)",
+            "traffic": traffic,
+            "kernel": "k_sweep_quad steady-state instance (half-sweep)" if args.kernel != 1 else
+                      "k_sweep_linear steady-state instance (half-sweep)",
             "bytes_per_update": BYTES_PER_UPDATE,
             "avg_launch_ms": round(avg_launch_ms, 5),
             "launches_timed": launches,

@@ -65,7 +65,24 @@ def build(jobs: int | None = None, out: str = LIB, build_dir: str = BUILD, extra
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, out)
+    if out == LIB:
+        build_dropin_test()
     return out
+
+
+def build_dropin_test() -> str:
+    """Link tests/cpp/dropin_stereo.cc (a C++ caller of the drop-in API)
+    against the library with the host compiler."""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_stereo.cc")
+    os.makedirs(os.path.join(PKG, "bin"), exist_ok=True)
+    exe = os.path.join(PKG, "bin", "dropin_stereo")
+    cmd = ["g++", "-std=c++17", "-O2", f"-I{INCLUDE}", src, f"-L{PKG}", "-lginf_bp",
+           "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("drop-in test link failed")
+    return exe
 
 
 if __name__ == "__main__":
