@@ -25,6 +25,8 @@ EXPORTS = [
     "ginf_hbp_host", "ginf_energy_f64", "ginf_bp_plan_dump_msgs", "ginf_bp_plan_level_dims",
     "ginf_gen_stereo", "ginf_last_error", "ginf_version", "ginf_bp_plan_set_sweep_timing",
     "ginf_bp_plan_sweep_stats", "ginf_bp_plan_updates_per_run", "ginf_bp_plan_dump_data",
+    "ginf_batch_create", "ginf_batch_run", "ginf_batch_set_tiles", "ginf_batch_updates_per_run",
+    "ginf_batch_dump_msgs", "ginf_batch_destroy",
 ]
 
 

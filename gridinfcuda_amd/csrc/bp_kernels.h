@@ -34,7 +34,7 @@ inline StateLayout state_layout(const SweepShape &sh, int L) {
 }
 
 // Label capacities built for the quad kernel.
-#define GINF_QUAD_LIST(X) X(32) X(64) X(96) X(128) X(256)
+#define GINF_QUAD_LIST(X) X(16) X(32) X(64) X(96) X(128) X(256)
 
 // Every (C, G) pair built for the O(L) kernel; L <= C * G.
 #define GINF_SHAPE_LIST(X) \
@@ -77,6 +77,17 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
                           const LevelGeom &cg, int C, int G, hipStream_t st);
 hipError_t launch_upsample(int dtype, const UpsampleArgs &u, hipStream_t st);
+// Batched persistent half-sweep (sweep_batch.hip): args[i] describes instance
+// i for the moving colour; items[k] = (instance, first slot, slots).
+struct BatchItem {
+    int inst;
+    int slot0;
+    int nslots;
+    int pad;
+};
+hipError_t launch_sweep_batch(int dtype, const SweepArgs *args_dev, const BatchItem *items_dev, int nitems, int grid,
+                              bool zeroin, bool labels, hipStream_t st);
+
 hipError_t launch_fill_u32(void *p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_zero_labels(int32_t *labels, int W, int H, int color, hipStream_t st);
 hipError_t launch_energy(int dtype, const void *data, const void *table, const int32_t *labels, int W, int H, int L,

@@ -59,6 +59,8 @@ struct SweepArgs {
     uint32_t in_plane_bytes; // same, source level
     uint32_t data_bytes;     // bytes of the data-cost array of the moving colour
     uint32_t in_span;        // bytes addressable from in_base
+    int lmax;                // label capacity of the quad kernel instance (batched launches)
+    int pad_;
 };
 
 }  // namespace ginf

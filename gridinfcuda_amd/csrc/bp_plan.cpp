@@ -538,3 +538,250 @@ int ginf_energy_f64(int width, int height, int num_labels, int dtype, const void
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many single-level instances advanced together.  Each half-sweep
+// is one persistent launch over a work list covering every instance; the
+// whole decode (relayout + all half-sweeps) is captured once into a HIP graph
+// and replayed.
+
+struct ginf_batch_s {
+    int n = 0;
+    int dtype = 0, norm = 0, iters = 0;
+    std::vector<ginf_bp_plan> plans;
+    std::vector<ginf::SweepArgs> args_host[2];
+    ginf::SweepArgs *args_dev = nullptr;  // [2][n]
+    ginf::BatchItem *items_dev = nullptr;
+    int nitems = 0;
+    int slots_per_item[4] = {256, 256, 256, 256};  // by label capacity 16, 32, 64, 128
+    int blocks_per_cu = 4;
+    int num_cu = 0;
+    hipStream_t cap = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<const void *> cap_data;
+    std::vector<int32_t *> cap_labels;
+};
+
+namespace {
+
+int lmax_class(int lmax) { return lmax <= 16 ? 0 : lmax <= 32 ? 1 : lmax <= 64 ? 2 : 3; }
+
+void batch_drop_graph(ginf_batch b) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->cap_data.clear();
+    b->cap_labels.clear();
+}
+
+int batch_build_items(ginf_batch b) {
+    std::vector<ginf::BatchItem> items;
+    for (int i = 0; i < b->n; ++i) {
+        const ginf_bp_plan p = b->plans[i];
+        const int ns = p->geom[0].nslots;
+        const int per = b->slots_per_item[lmax_class(p->shape.C)];
+        for (int s0 = 0; s0 < ns; s0 += per) items.push_back({i, s0, ns - s0 < per ? ns - s0 : per, 0});
+    }
+    // Largest instances' items interleaved with the rest: order by item index
+    // within instance so every instance is in flight early.
+    std::stable_sort(items.begin(), items.end(), [&](const ginf::BatchItem &x, const ginf::BatchItem &y) {
+        const int rx = x.slot0 / b->slots_per_item[lmax_class(b->plans[x.inst]->shape.C)];
+        const int ry = y.slot0 / b->slots_per_item[lmax_class(b->plans[y.inst]->shape.C)];
+        return rx < ry;
+    });
+    if (b->items_dev) { hipFree(b->items_dev); b->items_dev = nullptr; }
+    b->nitems = (int)items.size();
+    GINF_HIP_TRY(hipMalloc(&b->items_dev, sizeof(ginf::BatchItem) * (items.empty() ? 1 : items.size())));
+    if (!items.empty())
+        GINF_HIP_TRY(hipMemcpy(b->items_dev, items.data(), sizeof(ginf::BatchItem) * items.size(), hipMemcpyHostToDevice));
+    return GINF_OK;
+}
+
+int batch_capture(ginf_batch b, const void *const *data_dev, int32_t *const *labels_dev) {
+    using namespace ginf;
+    batch_drop_graph(b);
+    for (int c = 0; c < 2; ++c)
+        for (int i = 0; i < b->n; ++i) {
+            b->args_host[c][i].labels = labels_dev[i];
+        }
+    GINF_HIP_TRY(hipMemcpy(b->args_dev, b->args_host[0].data(), sizeof(SweepArgs) * b->n, hipMemcpyHostToDevice));
+    GINF_HIP_TRY(hipMemcpy(b->args_dev + b->n, b->args_host[1].data(), sizeof(SweepArgs) * b->n, hipMemcpyHostToDevice));
+    const int N = b->iters;
+    hipStream_t st = b->cap;
+    GINF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = GINF_OK;
+    auto fail = [&](hipError_t e) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(st, &g);
+        if (g) hipGraphDestroy(g);
+        return set_error(GINF_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
+    };
+    for (int i = 0; i < b->n && rc == GINF_OK; ++i) {
+        const ginf_bp_plan p = b->plans[i];
+        const StateLayout ly = state_layout(p->shape, p->desc.num_labels);
+        hipError_t e = launch_relayout_pyr(b->dtype, data_dev[i], p->desc.width, p->desc.height, p->desc.num_labels, ly,
+                                           p->geom[0], p->lv[0].data[0], p->lv[0].data[1], false, p->geom[0],
+                                           p->lv[0].data[0], p->lv[0].data[1], st);
+        if (e != hipSuccess) return fail(e);
+        if (N <= 1) {
+            if (N == 1) {
+                const int64_t per = (int64_t)ly.C * p->geom[0].nplane * ly.G;
+                for (int c = 0; c < 2; ++c)
+                    for (int k = 0; k < 4; ++k)
+                        if ((e = launch_fill_u32(p->lv[0].msgs[c][k], per, 0u, st)) != hipSuccess) return fail(e);
+            }
+            for (int c = (N == 1 ? 0 : 0); c < (N == 1 ? 1 : 2); ++c)
+                if ((e = launch_zero_labels(labels_dev[i], p->desc.width, p->desc.height, c, st)) != hipSuccess)
+                    return fail(e);
+        }
+    }
+    const int grid = b->num_cu * b->blocks_per_cu;
+    for (int it = 0; it < N; ++it) {
+        const int color = (it + 1) & 1;
+        const bool zeroin = N >= 2 && it == 0;
+        const bool labels = it >= N - 2;
+        hipError_t e = launch_sweep_batch(b->dtype, b->args_dev + (size_t)color * b->n, b->items_dev, b->nitems, grid,
+                                          zeroin, labels, st);
+        if (e != hipSuccess) return fail(e);
+    }
+    hipGraph_t g = nullptr;
+    GINF_HIP_TRY(hipStreamEndCapture(st, &g));
+    b->graph = g;
+    GINF_HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    b->cap_data.assign(data_dev, data_dev + b->n);
+    b->cap_labels.assign(labels_dev, labels_dev + b->n);
+    return GINF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ginf_batch_create(int n, const ginf_bp_desc *descs, const void *const *tables_host, ginf_batch *out) {
+    if (!out || !descs || !tables_host || n < 1) return set_error(GINF_ERR_INVALID, "bad batch arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        const ginf_bp_desc &d = descs[i];
+        int rc = validate_desc(&d);
+        if (rc) return rc;
+        if (d.num_levels != 1) return set_error(GINF_ERR_UNSUPPORTED, "batch instance %d: num_levels must be 1", i);
+        if (!is_linear(d.sm_model)) return set_error(GINF_ERR_UNSUPPORTED, "batch instance %d: Potts/TL only", i);
+        if (d.num_labels > 128) return set_error(GINF_ERR_UNSUPPORTED, "batch instance %d: L > 128", i);
+        // the batched kernel is built for label capacities 16, 32, 64, 96 and 128
+        if (d.dtype != descs[0].dtype || d.norm != descs[0].norm || d.iters_per_level != descs[0].iters_per_level)
+            return set_error(GINF_ERR_INVALID, "batch instances must share dtype, norm and iterations");
+    }
+    ginf_batch b = new (std::nothrow) ginf_batch_s;
+    if (!b) return set_error(GINF_ERR_NOMEM, "host allocation failed");
+    b->n = n;
+    b->dtype = descs[0].dtype;
+    b->norm = descs[0].norm;
+    b->iters = descs[0].iters_per_level;
+    auto cleanup = [&](int rc) {
+        ginf_batch_destroy(b);
+        return rc;
+    };
+    for (int i = 0; i < n; ++i) {
+        ginf_bp_desc d = descs[i];
+        d.kernel = GINF_KERNEL_QUAD;
+        d.lanes_per_node = 0;
+        ginf_bp_plan p = nullptr;
+        int rc = ginf_bp_plan_create(&d, tables_host[i], &p);
+        if (rc) return cleanup(rc);
+        b->plans.push_back(p);
+        for (int c = 0; c < 2; ++c) {
+            ginf::SweepArgs a{};
+            const ginf::LevelGeom &g = p->geom[0];
+            const ginf::StateLayout ly = ginf::state_layout(p->shape, d.num_labels);
+            a.data = p->lv[0].data[c];
+            for (int k = 0; k < 4; ++k) { a.out[k] = p->lv[0].msgs[c][k]; a.in[0][k] = p->lv[0].msgs[0][k]; a.in[1][k] = p->lv[0].msgs[1][k]; }
+            a.stride = a.in_stride = g.nplane * ly.G;
+            a.w = g.w; a.h = g.h; a.whp = g.whp; a.nslots = g.nslots; a.in_whp = g.whp;
+            a.color = c;
+            a.src = ginf::kSrcSame;
+            a.num_labels = d.num_labels;
+            a.norm = d.norm;
+            a.label_w = d.width;
+            a.s_f = p->s_f; a.tau_f = p->tau_f; a.s_i = p->s_i; a.tau_i = p->tau_i;
+            a.table = p->table_dev;
+            a.arr_bytes = a.data_bytes = a.in_arr_bytes = (uint32_t)p->arr_bytes[0];
+            a.plane_bytes = a.in_plane_bytes = (uint32_t)(g.nplane * ly.G * 4);
+            a.out_base = p->lv[0].msgs[c][0];
+            a.in_base = p->lv[0].msgs[1 - c][0];
+            a.in_span = (uint32_t)(4 * p->arr_bytes[0]);
+            a.lmax = p->shape.C;
+            b->args_host[c].push_back(a);
+        }
+    }
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&b->num_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (b->num_cu <= 0) b->num_cu = 256;
+    hipError_t e = hipMalloc(&b->args_dev, sizeof(ginf::SweepArgs) * 2 * n);
+    if (e != hipSuccess) return cleanup(set_error(GINF_ERR_NOMEM, "batch args: %s", hipGetErrorString(e)));
+    e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+    if (e != hipSuccess) return cleanup(set_error(GINF_ERR_HIP, "batch stream: %s", hipGetErrorString(e)));
+    int rc = batch_build_items(b);
+    if (rc) return cleanup(rc);
+    *out = b;
+    return GINF_OK;
+}
+
+int ginf_batch_set_tiles(ginf_batch b, const int *slots_per_item, int blocks_per_cu) {
+    if (!b || !slots_per_item) return set_error(GINF_ERR_INVALID, "NULL argument");
+    for (int k = 0; k < 4; ++k)
+        if (slots_per_item[k] < 64 || slots_per_item[k] % 64) return set_error(GINF_ERR_INVALID, "slots per item must be a positive multiple of 64");
+    if (blocks_per_cu < 1 || blocks_per_cu > 64) return set_error(GINF_ERR_INVALID, "blocks_per_cu %d", blocks_per_cu);
+    for (int k = 0; k < 4; ++k) b->slots_per_item[k] = slots_per_item[k];
+    b->blocks_per_cu = blocks_per_cu;
+    batch_drop_graph(b);
+    return batch_build_items(b);
+}
+
+int ginf_batch_run(ginf_batch b, const void *const *data_dev, int32_t *const *labels_dev, void *stream) {
+    if (!b || !data_dev || !labels_dev) return set_error(GINF_ERR_INVALID, "NULL argument");
+    for (int i = 0; i < b->n; ++i)
+        if (!data_dev[i] || !labels_dev[i]) return set_error(GINF_ERR_INVALID, "NULL pointer for instance %d", i);
+    bool same = b->exec && std::equal(b->cap_data.begin(), b->cap_data.end(), data_dev) &&
+                std::equal(b->cap_labels.begin(), b->cap_labels.end(), labels_dev);
+    if (!same) {
+        // The graph is captured on a private stream; order it after prior work on `stream`.
+        GINF_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+        int rc = batch_capture(b, data_dev, labels_dev);
+        if (rc) return rc;
+    }
+    GINF_HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return GINF_OK;
+}
+
+int ginf_batch_updates_per_run(ginf_batch b, double *updates) {
+    if (!b || !updates) return set_error(GINF_ERR_INVALID, "NULL argument");
+    double u = 0.0;
+    for (auto p : b->plans) {
+        double v = 0.0;
+        ginf_bp_plan_updates_per_run(p, &v);
+        u += v;
+    }
+    *updates = u;
+    return GINF_OK;
+}
+
+int ginf_batch_dump_msgs(ginf_batch b, int instance, void *msgs_host) {
+    if (!b || instance < 0 || instance >= b->n) return set_error(GINF_ERR_INVALID, "bad instance");
+    return ginf_bp_plan_dump_msgs(b->plans[instance], 0, msgs_host);
+}
+
+int ginf_batch_destroy(ginf_batch b) {
+    if (!b) return GINF_OK;
+    batch_drop_graph(b);
+    if (b->cap) hipStreamDestroy(b->cap);
+    if (b->args_dev) hipFree(b->args_dev);
+    if (b->items_dev) hipFree(b->items_dev);
+    for (auto p : b->plans) ginf_bp_plan_destroy(p);
+    delete b;
+    return GINF_OK;
+}
+
+}  // extern "C"

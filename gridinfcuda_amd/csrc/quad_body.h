@@ -1,0 +1,159 @@
+// Per-node body of the quad half-sweep (one lane per node and direction),
+// shared by the single-grid kernel (sweep_quad.hip) and the batched
+// persistent kernel (sweep_batch.hip).
+#pragma once
+
+#include "sweep_common.h"
+
+// Cache policy of the streaming buffer accesses (0 = default; 2 = non-temporal).
+// Outboxes are not read again within the half-sweep, so they are stored
+// non-temporally (measured faster on 1920x1080x128); loads keep the default.
+#ifndef GINF_QUAD_LOAD_POLICY
+#define GINF_QUAD_LOAD_POLICY 0
+#endif
+#ifndef GINF_QUAD_STORE_POLICY
+#define GINF_QUAD_STORE_POLICY 2
+#endif
+
+namespace ginf {
+namespace {
+
+// Broadcast lane k of each group of four lanes (DPP quad_perm [k,k,k,k]).
+template <int K>
+__device__ __forceinline__ float quad_bcast(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), K * 0x55, 0xF, 0xF, false));
+}
+template <int K>
+__device__ __forceinline__ int quad_bcast(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, K * 0x55, 0xF, 0xF, false);
+}
+
+// Red-black half-sweep for the O(L) models with four lanes per node, lane d
+// owning direction d: every lane forms the belief in the reference order
+// (D + in_S + in_E + in_N + in_W, the four incoming messages shared across the
+// quad by DPP broadcasts) and then runs bpSendMsg for its own direction with
+// the reference's sequential scans and left-fold normaliser, so every
+// operation -- FP32 mean mode included -- rounds exactly as src/bp.cc:26-54.
+//   LMAX  register capacity in labels; FULL: L == LMAX (no label guards)
+// Layout: label quads, element (l, slot) at ((l/4) * nplane + slot) * 4 + l%4,
+// so one lane moves 4 labels (16 B) per access and 16 consecutive nodes of a
+// wave cover 256 contiguous bytes of every array.  Offsets are 32-bit and
+// carried entirely in the VGPR offset (soffset = 0): with the plane offset in
+// soffset, the compiler reused SGPRs that a v_cmp in the labelling code had
+// just written, and the memory instruction then intermittently read the wrong
+// offset (nondeterministic message corruption, measured on gfx950).
+template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL>
+__device__ __forceinline__ void quad_node(const SweepArgs &a, const int slot, const int d) {
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    constexpr int LQ = LMAX / 4;
+    if (slot >= a.nslots) return;
+    const int y = slot / a.whp;
+    const int x = 2 * (slot - y * a.whp) + ((y + a.color) & 1);
+    if (y < 1 || y > a.h - 2 || x < 1 || x > a.w - 2) return;  // whole quads leave together
+    const int L = FULL ? LMAX : a.num_labels;
+    const T s = Num<T>::s(a);
+    const T tau = Num<T>::tau(a);
+
+    const __amdgpu_buffer_rsrc_t drs = __builtin_amdgcn_make_buffer_rsrc((void *)a.data, 0, a.data_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t irs = __builtin_amdgcn_make_buffer_rsrc((void *)a.in_base, 0, a.in_span, 0x00020000);
+    const __amdgpu_buffer_rsrc_t ors = __builtin_amdgcn_make_buffer_rsrc(a.out_base, 0, 4u * a.arr_bytes, 0x00020000);
+    const uint32_t self_off = (uint32_t)slot * 16u;
+    uint32_t in_off = 0;
+    if (!ZEROIN) {
+        int nx = x + kDX[d], ny = y + kDY[d];
+        int csel;
+        if (a.src == kSrcCoarse) { nx >>= 1; ny >>= 1; csel = (nx + ny) & 1; }
+        else csel = 0;  // in_base already points at the partner colour's arrays
+        const int od = (d + 2) & 3;
+        in_off = (uint32_t)(csel * 4 + od) * a.in_arr_bytes + (uint32_t)(ny * a.in_whp + (nx >> 1)) * 16u;
+    }
+    const uint32_t out_off = (uint32_t)d * a.arr_bytes + self_off;
+
+    T f[LMAX];
+    T mu = T(0);
+    // Labels: lane d tracks the belief minimum over labels l = 4j + d only
+    // (strict <, so the lowest such label wins); the quad combines the four
+    // residues at the end.
+    T best = T(0);
+    int arg = -1;
+#pragma unroll
+    for (int j = 0; j < LQ; ++j) {
+        if (FULL || 4 * j < L) {
+            v4 m4 = v4(T(0));
+            if (!ZEROIN)
+                m4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(irs, in_off + (uint32_t)j * a.in_plane_bytes, 0, GINF_QUAD_LOAD_POLICY));
+            const v4 d4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(drs, self_off + (uint32_t)j * a.plane_bytes, 0, GINF_QUAD_LOAD_POLICY));
+            // Beliefs of the four labels, D + in_S + in_E + in_N + in_W added
+            // left to right (a + b == b + a exactly, so the broadcast operand
+            // may come first).  The four chains are interleaved so that no DPP
+            // read waits on the add just before it.
+            v4 b4;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b4[r] = quad_bcast<0>(m4[r]) + d4[r];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b4[r] = quad_bcast<1>(m4[r]) + b4[r];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b4[r] = quad_bcast<2>(m4[r]) + b4[r];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b4[r] = quad_bcast<3>(m4[r]) + b4[r];
+            T bsel = T(0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int l = 4 * j + r;
+                if (FULL || l < L) {
+                    const T m = m4[r];
+                    const T b = b4[r];
+                    if (LABELS && r == d) bsel = b;
+                    const T h = b - m;
+                    if (l == 0) { f[0] = h; mu = h; }
+                    else { f[l] = tmin(h, f[l - 1] + s); mu = tmin(mu, h); }
+                }
+            }
+#ifdef GINF_DEBUG_NO_ARGMIN
+            if (false) {
+#else
+            if (LABELS && (FULL || 4 * j + d < L)) {
+#endif
+                if (arg < 0 || bsel < best) { best = bsel; arg = 4 * j + d; }
+            }
+        }
+    }
+#pragma unroll
+    for (int l = LMAX - 2; l >= 0; --l)
+        if (FULL || l <= L - 2) f[l] = tmin(f[l], f[l + 1] + s);
+    const T cap = mu + tau;
+    T acc = T(0), mn = T(0);
+#pragma unroll
+    for (int l = 0; l < LMAX; ++l) {
+        if (FULL || l < L) {
+            f[l] = tmin(f[l], cap);
+            acc = acc + f[l];
+            mn = (l == 0) ? f[0] : tmin(mn, f[l]);
+        }
+    }
+    const T nu = (a.norm == 0) ? acc / T(L) : mn;
+#pragma unroll
+    for (int j = 0; j < LQ; ++j) {
+        if (FULL || 4 * j < L) {
+            v4 o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) o[r] = (FULL || 4 * j + r < L) ? f[4 * j + r] - nu : T(0);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, o),
+                                                   ors, out_off + (uint32_t)j * a.plane_bytes, 0, GINF_QUAD_STORE_POLICY);
+        }
+    }
+    if (LABELS) {
+        // Combine the residues: smaller belief wins, ties go to the lower label.
+#pragma unroll
+        for (int k = 1; k < 4; k <<= 1) {
+            const T ob = __shfl_xor(best, k, 4);
+            const int oa = __shfl_xor(arg, k, 4);
+            const bool take = oa >= 0 && (arg < 0 || ob < best || (!(best < ob) && oa < arg));
+            if (take) { best = ob; arg = oa; }
+        }
+        if (d == 0) a.labels[(int64_t)y * a.label_w + x] = arg;
+    }
+}
+
+}  // namespace
+}  // namespace ginf

@@ -134,6 +134,23 @@ int ginf_gen_stereo(uint64_t seed, int width, int height, int num_labels, int nu
                     int trunc, uint8_t *left, uint8_t *right, int32_t *disp, float *cost_f32,
                     int32_t *cost_i32);
 
+/* Batches (no reference counterpart; the batched workload of the benchmark
+ * configurations): n independent single-level MRFs of any sizes and label
+ * counts (Potts / truncated linear, L <= 128) sharing dtype, normalisation
+ * and iteration count.  Each half-sweep of all instances is one persistent
+ * launch over a work list; the whole decode is captured once into a HIP
+ * graph (re-captured when the data/label pointers change) and replayed.
+ * Results equal n separate ginf_bp_plan_run calls with GINF_KERNEL_QUAD. */
+typedef struct ginf_batch_s *ginf_batch;
+int ginf_batch_create(int n, const ginf_bp_desc *descs, const void *const *tables_host, ginf_batch *out);
+int ginf_batch_run(ginf_batch batch, const void *const *data_dev, int32_t *const *labels_dev, void *stream);
+/* Tile shape: slots per work item for label capacities 16, 32, 64, 128
+ * (multiples of 64) and persistent blocks per compute unit. */
+int ginf_batch_set_tiles(ginf_batch batch, const int *slots_per_item, int blocks_per_cu);
+int ginf_batch_updates_per_run(ginf_batch batch, double *node_label_updates);
+int ginf_batch_dump_msgs(ginf_batch batch, int instance, void *msgs_host);
+int ginf_batch_destroy(ginf_batch batch);
+
 const char *ginf_last_error(void);
 
 /* Library version string. */

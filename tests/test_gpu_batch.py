@@ -1,0 +1,85 @@
+"""Batched BP: every instance of a mixed batch (sizes, label counts, Potts /
+truncated linear) equals the oracle run on that instance alone; results do
+not depend on the tile shape; graph replays are identical."""
+import numpy as np
+import pytest
+
+from gpu_util import torch_cuda, say
+from gridinfcuda_amd import _lib as G
+from gridinfcuda_amd import batch as B
+
+pytestmark = pytest.mark.gpu
+
+
+def small_instances(n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        L = B.LABEL_CLASSES[i % 4] if i < 8 else int(rng.choice([5, 16, 17, 32, 50, 64, 100, 128]))
+        W, H = int(rng.integers(3, 70)), int(rng.integers(3, 50))
+        potts = i % 2 == 0
+        out.append(dict(index=i, seed=900 + i, W=W, H=H, L=L, model=G.SM_POTTS if potts else G.SM_TRUNC_LINEAR,
+                        table=B.table_potts(L, 15) if potts else B.table_trunc_linear(L, 8, 32)))
+    return out
+
+
+def run_batch(torch, bt, datas, inits, stream=None):
+    dd = [torch.from_numpy(d).cuda() for d in datas]
+    ll = [torch.from_numpy(i.copy()).cuda() for i in inits]
+    s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    bt.run([t.data_ptr() for t in dd], [t.data_ptr() for t in ll], s)
+    torch.cuda.synchronize()
+    return dd, ll
+
+
+@pytest.mark.parametrize("dt,norm,iters", [("float32", 0, 6), ("int32", 0, 5), ("float32", 1, 4), ("float32", 0, 2),
+                                           ("float32", 0, 1), ("int32", 0, 0)])
+def test_batch_matches_oracle(oracle_mod, dt, norm, iters):
+    torch = torch_cuda()
+    say(f"batch {dt} norm={norm} iters={iters}")
+    insts = small_instances(14, 5 + iters)
+    dtype = np.dtype(dt)
+    datas = [B.instance_costs(x, dtype) for x in insts]
+    rng = np.random.default_rng(1)
+    inits = [rng.integers(0, x["L"], size=(x["H"], x["W"])).astype(np.int32) for x in insts]
+    bt = B.make_batch(insts, iters=iters, norm=norm, dtype=dtype)
+    dd, ll = run_batch(torch, bt, datas, inits)
+    for k, x in enumerate(insts):
+        ref_lab, ref_msgs, _ = oracle_mod.hbp(datas[k], x["table"].astype(dtype), x["model"], 1, iters, norm=norm,
+                                              labels_init=inits[k], return_state=True)
+        np.testing.assert_array_equal(ll[k].cpu().numpy(), ref_lab, err_msg=f"instance {k}")
+        np.testing.assert_array_equal(bt.dump_msgs(k), ref_msgs, err_msg=f"instance {k}")
+
+
+def test_batch_tiles_and_replays_identical(oracle_mod):
+    torch = torch_cuda()
+    say("batch tiles / replays")
+    insts = small_instances(12, 77)
+    datas = [B.instance_costs(x) for x in insts]
+    bt = B.make_batch(insts, iters=7)
+    dd = [torch.from_numpy(d).cuda() for d in datas]
+    ll = [torch.zeros((x["H"], x["W"]), dtype=torch.int32, device="cuda") for x in insts]
+    ptrs = ([t.data_ptr() for t in dd], [t.data_ptr() for t in ll])
+    s = torch.cuda.current_stream().cuda_stream
+    refs = [oracle_mod.bp(datas[k], x["table"], x["model"], 7) for k, x in enumerate(insts)]
+    for tiles, bpc in [((256,) * 4, 4), ((64,) * 4, 2), ((1024, 64, 512, 128), 8), ((128,) * 4, 3)]:
+        bt.set_tiles(tiles, bpc)
+        for rep in range(3):
+            for t in ll:
+                t.fill_(-1)
+            _ = (dd[0] * 2).sum().item()  # eager work between replays
+            bt.run(*ptrs, s)
+            torch.cuda.synchronize()
+            for k in range(len(insts)):
+                lab = ll[k].cpu().numpy()
+                inner = lab[1:-1, 1:-1]
+                np.testing.assert_array_equal(inner, refs[k][1:-1, 1:-1], err_msg=f"tiles {tiles} rep {rep} inst {k}")
+
+
+def test_batch_rejects_bad_instances():
+    d = [G.make_desc(10, 10, 4, num_levels=2)]
+    with pytest.raises(G.GinfError):
+        B.Batch(d, [B.table_potts(4, 1)])
+    d = [G.make_desc(10, 10, 200)]
+    with pytest.raises(G.GinfError):
+        B.Batch(d, [B.table_potts(200, 1)])
