@@ -26,7 +26,10 @@ EXPORTS = [
     "ginf_gen_stereo", "ginf_last_error", "ginf_version", "ginf_bp_plan_set_sweep_timing",
     "ginf_bp_plan_sweep_stats", "ginf_bp_plan_updates_per_run", "ginf_bp_plan_dump_data",
     "ginf_batch_create", "ginf_batch_run", "ginf_batch_set_tiles", "ginf_batch_updates_per_run",
-    "ginf_batch_dump_msgs", "ginf_batch_destroy",
+    "ginf_batch_dump_msgs", "ginf_batch_destroy", "ginf_bp_plan_run_schedule", "ginf_bp_plan_seed",
+    "ginf_bp_plan_block_mean_msgs", "ginf_conv3x3_fwd", "ginf_conv3x3_bwd_weight", "ginf_conv3x3_flip_weights",
+    "ginf_relu_bwd", "ginf_mse", "ginf_adam", "ginf_to_bf16", "ginf_minpool_costs", "ginf_block_mean_msgs",
+    "ginf_stereo_costs",
 ]
 
 
@@ -77,6 +80,20 @@ def lib() -> ctypes.CDLL:
         L.ginf_bp_plan_set_sweep_timing.argtypes = [VP, I]
         L.ginf_bp_plan_sweep_stats.argtypes = [VP, P(ctypes.c_double), P(ctypes.c_int64), P(ctypes.c_double), I]
         L.ginf_bp_plan_updates_per_run.argtypes = [VP, P(ctypes.c_double)]
+        I64, F = ctypes.c_int64, ctypes.c_float
+        L.ginf_bp_plan_run_schedule.argtypes = [VP, VP, VP, I, P(I), I, VP]
+        L.ginf_bp_plan_seed.argtypes = [VP, I, VP, I, I, I, F, VP]
+        L.ginf_bp_plan_block_mean_msgs.argtypes = [VP, I, I, F, VP, VP]
+        L.ginf_conv3x3_fwd.argtypes = [VP, VP, VP, I, I, I, I, I, I, VP, VP, VP]
+        L.ginf_conv3x3_bwd_weight.argtypes = [VP, VP, I, I, I, I, I, VP, VP, VP]
+        L.ginf_conv3x3_flip_weights.argtypes = [VP, I, I, VP, VP]
+        L.ginf_relu_bwd.argtypes = [VP, VP, I64, VP, VP]
+        L.ginf_mse.argtypes = [VP, VP, I64, VP, VP, P(I), VP]
+        L.ginf_adam.argtypes = [VP, VP, VP, VP, VP, I64, F, F, F, F, I, F, VP]
+        L.ginf_to_bf16.argtypes = [VP, I64, VP, VP]
+        L.ginf_minpool_costs.argtypes = [VP, I, I, I, I, I, F, VP, VP]
+        L.ginf_block_mean_msgs.argtypes = [VP, I, I, I, I, F, VP, VP]
+        L.ginf_stereo_costs.argtypes = [VP, VP, I, I, I, I, VP, VP]
         L.ginf_last_error.restype = ctypes.c_char_p
         L.ginf_version.restype = ctypes.c_char_p
         for name in EXPORTS:
@@ -146,6 +163,20 @@ class Plan:
         out = np.zeros((L, h, w), np.float32 if self.desc.dtype == F32 else np.int32)
         check(lib().ginf_bp_plan_dump_data(self._h, level, out.ctypes.data_as(ctypes.c_void_p)))
         return out
+
+    def run_schedule(self, data_ptr: int, labels_ptr: int, start_level: int, iters, seeded: bool,
+                     stream: int = 0) -> None:
+        arr = (ctypes.c_int * self.desc.num_levels)(*([int(v) for v in iters] + [0] * (self.desc.num_levels - len(iters))))
+        check(lib().ginf_bp_plan_run_schedule(self._h, ctypes.c_void_p(data_ptr), ctypes.c_void_p(labels_ptr),
+                                              start_level, arr, 1 if seeded else 0, ctypes.c_void_p(stream)))
+
+    def seed(self, level: int, pred_ptr: int, pw: int, ph: int, shift: int, scale: float, stream: int = 0) -> None:
+        check(lib().ginf_bp_plan_seed(self._h, level, ctypes.c_void_p(pred_ptr), pw, ph, shift, scale,
+                                      ctypes.c_void_p(stream)))
+
+    def block_mean_msgs(self, level: int, f: int, scale: float, out_ptr: int, stream: int = 0) -> None:
+        check(lib().ginf_bp_plan_block_mean_msgs(self._h, level, f, scale, ctypes.c_void_p(out_ptr),
+                                                 ctypes.c_void_p(stream)))
 
     def set_sweep_timing(self, enable: bool) -> None:
         check(lib().ginf_bp_plan_set_sweep_timing(self._h, 1 if enable else 0))

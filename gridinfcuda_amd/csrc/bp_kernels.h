@@ -88,6 +88,13 @@ struct BatchItem {
 hipError_t launch_sweep_batch(int dtype, const SweepArgs *args_dev, const BatchItem *items_dev, int nitems, int grid,
                               bool zeroin, bool labels, hipStream_t st);
 
+// Seed a level's outboxes from an NHWC prediction (conv_net.hip).
+hipError_t launch_inject(const float *pred, void *const msgs[2][4], const LevelGeom &g, int C, int G, bool interleave,
+                         int L, int shift, int pw, int ph, float inv_scale, hipStream_t st);
+
+hipError_t launch_block_mean_state(void *const msgs[2][4], const LevelGeom &g, int C, int G, bool interleave, int L,
+                                   int f, float scale, float *out, hipStream_t st);
+
 hipError_t launch_fill_u32(void *p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_zero_labels(int32_t *labels, int W, int H, int color, hipStream_t st);
 hipError_t launch_energy(int dtype, const void *data, const void *table, const int32_t *labels, int W, int H, int L,

@@ -152,40 +152,39 @@ hipEvent_t take_event(ginf_bp_plan p) {
     return e;
 }
 
-int plan_run(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStream_t st) {
+// The HBP driver.  iters[t] half-sweeps run at level t for t <= start;
+// levels above `start` are skipped.  When `seeded`, level `start` begins from
+// the outboxes already in its arrays (written by ginf_bp_plan_seed) instead
+// of zeros.  ginf_bp_plan_run is start = levels-1, iters[t] = the
+// descriptor's count, not seeded: decodeHbp (src/bp.cc:86-180).
+int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStream_t st, int start,
+                const int *iters, bool seeded) {
     using namespace ginf;
     const ginf_bp_desc &d = p->desc;
     const StateLayout ly = state_layout(p->shape, d.num_labels);
-    const int dt = d.dtype, C = ly.C, G = ly.G, T = p->levels, N = d.iters_per_level;
+    const int dt = d.dtype, C = ly.C, G = ly.G, T = p->levels;
     const int W = d.width, H = d.height, L = d.num_labels;
     // Level 0 (and level 1) costs in one pass over the caller's array.
     const bool two = T > 1;
     GINF_HIP_TRY(launch_relayout_pyr(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0], p->lv[0].data[1], two,
                                      p->geom[two ? 1 : 0], p->lv[two ? 1 : 0].data[0], p->lv[two ? 1 : 0].data[1], st));
-    for (int t = 2; t < T; ++t)
+    for (int t = 2; t <= start; ++t)
         GINF_HIP_TRY(launch_pyramid(dt, p->lv[t - 1].data[0], p->lv[t - 1].data[1], p->geom[t - 1], p->lv[t].data[0],
                                     p->lv[t].data[1], p->geom[t], C, G, st));
-    if (N == 0) {
-        // Every belief stays zero: interior labels are 0 (src/bp.cc:168-173).
-        GINF_HIP_TRY(launch_zero_labels(labels_dev, W, H, 0, st));
-        GINF_HIP_TRY(launch_zero_labels(labels_dev, W, H, 1, st));
-        return GINF_OK;
-    }
-    // With two or more half-sweeps per level both colours move, so the first
-    // half-sweep of a level can read the level above directly (fused
-    // upsample), and the top level's first half-sweep reads zero messages.
-#ifdef GINF_DEBUG_NO_FUSE
-    const bool fused = false;
-#else
-    const bool fused = N >= 2;
-#endif
-    for (int t = T - 1; t >= 0; --t) {
+    for (int t = start; t >= 0; --t) {
         const LevelGeom &g = p->geom[t];
         const int64_t per = (int64_t)C * g.nplane * G;
+        const int N = iters[t];
+        // With two or more half-sweeps both colours move, so the first
+        // half-sweep can read the level above directly (fused upsample) or,
+        // at an unseeded top level, zero messages without reading them.
+        const bool top = t == start;
+        const bool fused = N >= 2 && !(top && seeded);
         if (!fused) {
-            if (t == T - 1) {
-                for (int c = 0; c < 2; ++c)
-                    for (int k = 0; k < 4; ++k) GINF_HIP_TRY(launch_fill_u32(p->lv[t].msgs[c][k], per, 0u, st));
+            if (top) {
+                if (!seeded)
+                    for (int c = 0; c < 2; ++c)
+                        for (int k = 0; k < 4; ++k) GINF_HIP_TRY(launch_fill_u32(p->lv[t].msgs[c][k], per, 0u, st));
             } else {
                 UpsampleArgs u{};
                 for (int c = 0; c < 2; ++c)
@@ -197,14 +196,20 @@ int plan_run(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStrea
                 GINF_HIP_TRY(launch_upsample(dt, u, st));
             }
         }
+        if (t == 0 && N < 2) {
+            // Interior nodes never updated at level 0 keep a zero belief: label 0
+            // (src/bp.cc:168-173).
+            GINF_HIP_TRY(launch_zero_labels(labels_dev, W, H, 0, st));
+            if (N == 0) GINF_HIP_TRY(launch_zero_labels(labels_dev, W, H, 1, st));
+        }
         for (int i = 0; i < N; ++i) {
             SweepArgs a{};
             const int color = (i + 1) & 1;  // (x + y + i) odd (src/bp.cc:150)
             a.data = p->lv[t].data[color];
             for (int k = 0; k < 4; ++k) a.out[k] = p->lv[t].msgs[color][k];
             bool zeroin = false;
-            const int src_level = (fused && i == 0 && t < T - 1) ? t + 1 : t;
-            if (fused && i == 0 && t == T - 1) zeroin = true;
+            const int src_level = (fused && i == 0 && !top) ? t + 1 : t;
+            if (fused && i == 0 && top) zeroin = true;
             for (int c = 0; c < 2; ++c)
                 for (int k = 0; k < 4; ++k) a.in[c][k] = p->lv[src_level].msgs[c][k];
             a.src = src_level == t ? kSrcSame : kSrcCoarse;
@@ -246,19 +251,20 @@ int plan_run(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStrea
                 if (tm.beg) GINF_HIP_TRY(hipEventRecord(tm.beg, st));
             }
             GINF_HIP_TRY(launch_sweep(dt, p->shape, a, zeroin, labels, st));
-#ifdef GINF_DEBUG_SYNC
-            GINF_HIP_TRY(hipDeviceSynchronize());
-#endif
             if (tm.beg && tm.end) {
                 GINF_HIP_TRY(hipEventRecord(tm.end, st));
                 p->timed.push_back(tm);
             }
         }
-        if (t == 0 && N == 1) GINF_HIP_TRY(launch_zero_labels(labels_dev, W, H, 0, st));
     }
     return GINF_OK;
 }
 
+
+int plan_run(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStream_t st) {
+    std::vector<int> iters((size_t)p->levels, p->desc.iters_per_level);
+    return plan_run_ex(p, data_dev, labels_dev, st, p->levels - 1, iters.data(), false);
+}
 }  // namespace
 
 extern "C" {
@@ -433,6 +439,43 @@ int ginf_bp_plan_level_dims(ginf_bp_plan plan, int level, int *w, int *h) {
 int ginf_bp_plan_run(ginf_bp_plan plan, const void *data_dev, int32_t *labels_dev, void *stream) {
     if (!plan || !data_dev || !labels_dev) return set_error(GINF_ERR_INVALID, "NULL argument");
     return plan_run(plan, data_dev, labels_dev, (hipStream_t)stream);
+}
+
+int ginf_bp_plan_run_schedule(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, int start_level,
+                              const int *iters, int seeded, void *stream) {
+    if (!p || !data_dev || !labels_dev || !iters) return set_error(GINF_ERR_INVALID, "NULL argument");
+    if (start_level < 0 || start_level >= p->levels) return set_error(GINF_ERR_INVALID, "start level %d", start_level);
+    for (int t = 0; t <= start_level; ++t)
+        if (iters[t] < 0) return set_error(GINF_ERR_INVALID, "iters[%d] < 0", t);
+    return plan_run_ex(p, data_dev, labels_dev, (hipStream_t)stream, start_level, iters, seeded != 0);
+}
+
+int ginf_bp_plan_seed(ginf_bp_plan p, int level, const float *pred_dev, int pw, int ph, int shift, float scale,
+                      void *stream) {
+    if (!p || !pred_dev || pw < 1 || ph < 1 || shift < 0 || shift > 8 || scale == 0.f)
+        return set_error(GINF_ERR_INVALID, "bad seed arguments");
+    if (level < 0 || level >= p->levels) return set_error(GINF_ERR_INVALID, "level %d out of range", level);
+    if (p->desc.dtype != GINF_F32) return set_error(GINF_ERR_UNSUPPORTED, "seeding needs FP32 messages");
+    const ginf::StateLayout ly = ginf::state_layout(p->shape, p->desc.num_labels);
+    void *m[2][4];
+    for (int c = 0; c < 2; ++c)
+        for (int d = 0; d < 4; ++d) m[c][d] = p->lv[level].msgs[c][d];
+    GINF_HIP_TRY(ginf::launch_inject(pred_dev, m, p->geom[level], ly.C, ly.G, ly.interleave, p->desc.num_labels, shift,
+                                     pw, ph, 1.f / scale, (hipStream_t)stream));
+    return GINF_OK;
+}
+
+int ginf_bp_plan_block_mean_msgs(ginf_bp_plan p, int level, int f, float scale, float *out_dev, void *stream) {
+    if (!p || !out_dev || f < 1) return set_error(GINF_ERR_INVALID, "bad arguments");
+    if (level < 0 || level >= p->levels) return set_error(GINF_ERR_INVALID, "level %d out of range", level);
+    if (p->desc.dtype != GINF_F32) return set_error(GINF_ERR_UNSUPPORTED, "FP32 plans only");
+    const ginf::StateLayout ly = ginf::state_layout(p->shape, p->desc.num_labels);
+    void *m[2][4];
+    for (int c = 0; c < 2; ++c)
+        for (int d = 0; d < 4; ++d) m[c][d] = p->lv[level].msgs[c][d];
+    GINF_HIP_TRY(ginf::launch_block_mean_state(m, p->geom[level], ly.C, ly.G, ly.interleave, p->desc.num_labels, f,
+                                               scale, out_dev, (hipStream_t)stream));
+    return GINF_OK;
 }
 
 int ginf_bp_plan_dump_msgs(ginf_bp_plan p, int level, void *msgs_host) {

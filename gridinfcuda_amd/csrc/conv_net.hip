@@ -1,0 +1,478 @@
+// BF16 3x3 convolution kernels on MFMA (gfx950 v_mfma_f32_16x16x32_bf16,
+// FP32 accumulate) for the learned warm start, plus the small elementwise
+// kernels around them (pooling, ReLU backward, loss, Adam, weight
+// preparation, message injection into the BP state).  No reference
+// counterpart: the warm start is a project-owned component.
+//
+// Tensors are NHWC (channels fastest); weights are [Cout][3][3][Cin].
+// MFMA operand layout (16x16x32): lane l holds A[l%16][8*(l/16) .. +7] and
+// B[8*(l/16) .. +7][l%16]; the result lane l holds D[4*(l/16) + i][l%16],
+// i = 0..3.  A rows are pixels, B columns output channels, K runs over
+// (kernel tap, input channel) in steps of 32 channels.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ginf_bp.h"
+#include "bp_layout.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+
+// Round-to-nearest-even float -> bf16 (NaN kept quiet).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
+}
+
+// y = conv3x3(x, w) + bias (zero padding 1), optional ReLU; each wave owns
+// 16 consecutive pixels (flattened n,h,w) and all Cout = 16*CT channels.
+template <int CT>
+__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
+                                                 const float *__restrict__ bias, int N, int H, int W, int Cin, int relu,
+                                                 uint16_t *__restrict__ yb, float *__restrict__ yf) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+    if (p0 >= P) return;
+    const int m = lane & 15, kg = lane >> 4;
+    const int64_t p = p0 + m;
+    const bool valid = p < P;
+    const int n = valid ? (int)(p / ((int64_t)H * W)) : 0;
+    const int rem = valid ? (int)(p - (int64_t)n * H * W) : 0;
+    const int h = rem / W, wc = rem - (rem / W) * W;
+    const int Cout = 16 * CT;
+    f4 acc[CT];
+#pragma unroll
+    for (int t = 0; t < CT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero = __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
+    for (int kh = 0; kh < 3; ++kh) {
+        for (int kw = 0; kw < 3; ++kw) {
+            const int hh = h + kh - 1, ww = wc + kw - 1;
+            const bool inb = valid && hh >= 0 && hh < H && ww >= 0 && ww < W;
+            const uint16_t *xp = x + (((int64_t)n * H + (inb ? hh : 0)) * W + (inb ? ww : 0)) * Cin + kg * 8;
+            const uint16_t *wp = w + ((int64_t)m * 9 + kh * 3 + kw) * Cin + kg * 8;
+            for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
+                const bf16x8 a = inb ? load8(xp + ci0) : zero;
+#pragma unroll
+                for (int t = 0; t < CT; ++t) {
+                    const bf16x8 b = load8(wp + (int64_t)t * 16 * 9 * Cin + ci0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+        const int co = t * 16 + m;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t pp = p0 + kg * 4 + i;
+            if (pp >= P) continue;
+            float v = acc[t][i] + bv;
+            if (relu) v = v > 0.f ? v : 0.f;
+            if (yb) yb[pp * Cout + co] = f2bf(v);
+            if (yf) yf[pp * Cout + co] = v;
+        }
+    }
+}
+
+// dW[co][kh][kw][ci] = sum_p dz[p][co] * x[p + (kh-1, kw-1)][ci]: one block per
+// (16-channel co tile, 16-channel ci tile, tap); its four waves split the
+// pixels and add their partial sums in LDS (fixed order, deterministic).
+__global__ __launch_bounds__(256) void k_conv3x3_dw(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dz,
+                                                    int N, int H, int W, int Cin, int Cout, float *__restrict__ dw) {
+    __shared__ float red[4][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ct = Cout / 16, it = Cin / 16;
+    int b = blockIdx.x;
+    const int tap = b % 9; b /= 9;
+    const int ti = b % it; b /= it;
+    const int to = b;
+    const int kh = tap / 3, kw = tap % 3;
+    const int m = lane & 15, kg = lane >> 4;
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t HW = (int64_t)H * W;
+    f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t q0 = (int64_t)wave * 32; q0 < P; q0 += 4 * 32) {
+        uint16_t av[8], bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int64_t p = q0 + kg * 8 + j;
+            uint16_t a = 0, bb = 0;
+            if (p < P) {
+                a = dz[p * Cout + to * 16 + m];
+                const int n = (int)(p / HW);
+                const int rem = (int)(p - n * HW);
+                const int hh = rem / W + kh - 1, ww = rem % W + kw - 1;
+                if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+                    bb = x[(((int64_t)n * H + hh) * W + ww) * Cin + ti * 16 + m];
+            }
+            av[j] = a;
+            bv[j] = bb;
+        }
+        bf16x8 A, B;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            A[j] = __builtin_bit_cast(__bf16, av[j]);
+            B[j] = __builtin_bit_cast(__bf16, bv[j]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wave][(kg * 4 + i) * 16 + m] = acc[i];
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        const int r = threadIdx.x / 16, c = threadIdx.x % 16;  // r: co within tile, c: ci within tile
+        const float v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+        const int co = to * 16 + r, ci = ti * 16 + c;
+        dw[((int64_t)co * 9 + tap) * Cin + ci] = v;
+    }
+}
+
+// db[co] = sum_p dz[p][co] (one block per channel, fixed-order tree).
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ dz, int64_t P, int Cout, float *db) {
+    __shared__ float red[256];
+    const int co = blockIdx.x;
+    float acc = 0.f;
+    for (int64_t p = threadIdx.x; p < P; p += 256) acc += bf2f(dz[p * Cout + co]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[co] = red[0];
+}
+
+// dz = bf16(dy * (y > 0)) for a ReLU layer (y its output), bf16(dy) otherwise.
+__global__ void k_relu_bwd(const float *__restrict__ dy, const uint16_t *__restrict__ y, int64_t n, uint16_t *dz) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float g = dy[i];
+        if (y && !(bf2f(y[i]) > 0.f)) g = 0.f;
+        dz[i] = f2bf(g);
+    }
+}
+
+// Mean squared error over n elements: loss partials and grad = 2 (o - t) / n.
+__global__ __launch_bounds__(256) void k_mse(const float *__restrict__ o, const float *__restrict__ t, int64_t n,
+                                             float *__restrict__ grad, float *__restrict__ partial) {
+    __shared__ float red[256];
+    float acc = 0.f;
+    const float inv = 2.f / (float)n;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float d = o[i] - t[i];
+        acc += d * d;
+        grad[i] = inv * d;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Adam on FP32 master weights, refreshing the BF16 copy used by the kernels.
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m1,
+                       float *__restrict__ m2, uint16_t *__restrict__ wb, int64_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, float wd) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i] + wd * w[i];
+        const float a = b1 * m1[i] + (1.f - b1) * gi;
+        const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+        m1[i] = a;
+        m2[i] = b;
+        const float v = w[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+        w[i] = v;
+        if (wb) wb[i] = f2bf(v);
+    }
+}
+
+// fp32 -> bf16 copy.
+__global__ void k_to_bf16(const float *__restrict__ s, int64_t n, uint16_t *__restrict__ d) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = f2bf(s[i]);
+}
+
+// wf[ci][kh][kw][co] = w[co][2-kh][2-kw][ci]: the flipped, transposed kernel
+// whose forward convolution is the input gradient of w.
+__global__ void k_flip_weights(const uint16_t *__restrict__ w, int Cout, int Cin, uint16_t *__restrict__ wf) {
+    const int64_t n = (int64_t)Cout * 9 * Cin;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int ci = (int)(i % Cin);
+        const int tap = (int)((i / Cin) % 9);
+        const int co = (int)(i / (Cin * 9));
+        const int kh = tap / 3, kw = tap % 3;
+        wf[((int64_t)ci * 9 + (2 - kh) * 3 + (2 - kw)) * Cout + co] = w[i];
+    }
+}
+
+// Warm-start input: costs [L][H][W] (reference layout) min-pooled over f x f
+// blocks, scaled, written NHWC bf16 [h][w][L] for instance slot `n`.
+template <typename T>
+__global__ void k_minpool(const T *__restrict__ d, int W, int H, int L, int f, float scale, int w4, int h4,
+                          uint16_t *__restrict__ out) {
+    const int64_t n = (int64_t)w4 * h4 * L;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int l = (int)(i % L);
+        const int X = (int)((i / L) % w4);
+        const int Y = (int)(i / ((int64_t)L * w4));
+        float mn = 3.4e38f;
+        for (int yy = Y * f; yy < Y * f + f && yy < H; ++yy)
+            for (int xx = X * f; xx < X * f + f && xx < W; ++xx) {
+                const float v = (float)d[((int64_t)l * H + yy) * W + xx];
+                mn = v < mn ? v : mn;
+            }
+        out[i] = f2bf(mn * scale);
+    }
+}
+
+// Training target: level-0 outboxes (oracle layout [l][d][y][x]) averaged over
+// f x f blocks, scaled, NHWC f32 [h][w][4L] with channel d*L + l.
+__global__ void k_block_mean_msgs(const float *__restrict__ m, int W, int H, int L, int f, float scale, int w4,
+                                  int h4, float *__restrict__ out) {
+    const int64_t n = (int64_t)w4 * h4 * 4 * L;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % (4 * L));
+        const int X = (int)((i / (4 * L)) % w4);
+        const int Y = (int)(i / ((int64_t)4 * L * w4));
+        const int d = ch / L, l = ch % L;
+        float acc = 0.f;
+        int cnt = 0;
+        for (int yy = Y * f; yy < Y * f + f && yy < H; ++yy)
+            for (int xx = X * f; xx < X * f + f && xx < W; ++xx) {
+                acc += m[(((int64_t)l * 4 + d) * H + yy) * W + xx];
+                ++cnt;
+            }
+        out[i] = cnt ? acc / cnt * scale : 0.f;
+    }
+}
+
+// Seed the outboxes of one plan level from a prediction at 1/2^shift of its
+// resolution: interior node (x, y) gets pred[(y >> shift)][(x >> shift)][d*L + l]
+// / scale; border outboxes stay 0.
+struct InjectArgs {
+    void *msgs[2][4];
+    ginf::LevelGeom g;
+    int C, G, interleave, L, shift, pw, ph;
+    float inv_scale;
+};
+
+__global__ void k_inject(const float *__restrict__ pred, InjectArgs a) {
+    const int64_t n = (int64_t)a.g.w * a.g.h * 4 * a.L;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int l = (int)(i % a.L);
+        const int d = (int)((i / a.L) % 4);
+        const int64_t node = i / (4 * a.L);
+        const int x = (int)(node % a.g.w), y = (int)(node / a.g.w);
+        if (x < 1 || y < 1 || x > a.g.w - 2 || y > a.g.h - 2) continue;
+        int X = x >> a.shift, Y = y >> a.shift;
+        X = X < a.pw ? X : a.pw - 1;
+        Y = Y < a.ph ? Y : a.ph - 1;
+        const float v = pred[((int64_t)Y * a.pw + X) * 4 * a.L + d * a.L + l] * a.inv_scale;
+        const int c = (x + y) & 1;
+        const int64_t slot = (int64_t)y * a.g.whp + (x >> 1);
+        const int q = a.interleave ? l % a.G : l / a.C;
+        const int ii = a.interleave ? l / a.G : l - q * a.C;
+        ((float *)a.msgs[c][d])[((int64_t)ii * a.g.nplane + slot) * a.G + q] = v;
+    }
+}
+
+// Block mean of a plan level's outboxes, read from the state layout.
+__global__ void k_block_mean_state(InjectArgs a, int f, float scale, float *__restrict__ out) {
+    const int w4 = (a.g.w + f - 1) / f, h4 = (a.g.h + f - 1) / f;
+    const int64_t n = (int64_t)w4 * h4 * 4 * a.L;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % (4 * a.L));
+        const int X = (int)((i / (4 * a.L)) % w4);
+        const int Y = (int)(i / ((int64_t)4 * a.L * w4));
+        const int d = ch / a.L, l = ch % a.L;
+        const int q = a.interleave ? l % a.G : l / a.C;
+        const int ii = a.interleave ? l / a.G : l - q * a.C;
+        float acc = 0.f;
+        int cnt = 0;
+        for (int y = Y * f; y < Y * f + f && y < a.g.h; ++y)
+            for (int x = X * f; x < X * f + f && x < a.g.w; ++x) {
+                const int c = (x + y) & 1;
+                const int64_t slot = (int64_t)y * a.g.whp + (x >> 1);
+                acc += ((const float *)a.msgs[c][d])[((int64_t)ii * a.g.nplane + slot) * a.G + q];
+                ++cnt;
+            }
+        out[i] = cnt ? acc / cnt * scale : 0.f;
+    }
+}
+
+// Stereo data costs from an image pair, the cost rule of ginf_gen_stereo:
+// min(|left(x,y) - right(x-l,y)|, trunc), trunc where x - l < 0.
+__global__ void k_stereo_costs(const uint8_t *__restrict__ left, const uint8_t *__restrict__ right, int W, int H,
+                               int L, int trunc, float *__restrict__ out) {
+    const int64_t n = (int64_t)W * H * L;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % W);
+        const int y = (int)((i / W) % H);
+        const int l = (int)(i / ((int64_t)W * H));
+        int c = trunc;
+        if (x - l >= 0) {
+            c = (int)left[(int64_t)y * W + x] - (int)right[(int64_t)y * W + x - l];
+            c = c < 0 ? -c : c;
+            c = c < trunc ? c : trunc;
+        }
+        out[i] = (float)c;
+    }
+}
+
+inline unsigned nblk(int64_t n, int per = 256) {
+    int64_t b = (n + per - 1) / per;
+    return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+}  // namespace
+
+extern "C" {
+
+int ginf_conv3x3_fwd(const uint16_t *x, const uint16_t *w, const float *bias, int N, int H, int W, int Cin, int Cout,
+                     int relu, uint16_t *y_bf16, float *y_f32, void *stream) {
+    if (!x || !w || (!y_bf16 && !y_f32) || N < 1 || H < 1 || W < 1 || Cin % 32 || Cout % 16 || Cout < 16 ||
+        Cout > 512)
+        return GINF_ERR_INVALID;
+    const int64_t P = (int64_t)N * H * W;
+    const dim3 grid((unsigned)((P + 63) / 64)), block(256);
+    hipStream_t st = (hipStream_t)stream;
+    switch (Cout / 16) {
+#define GINF_CV(CT) case CT: hipLaunchKernelGGL((k_conv3x3<CT>), grid, block, 0, st, x, w, bias, N, H, W, Cin, relu, y_bf16, y_f32); break;
+        GINF_CV(1) GINF_CV(2) GINF_CV(4) GINF_CV(6) GINF_CV(8) GINF_CV(12) GINF_CV(16) GINF_CV(24) GINF_CV(32)
+#undef GINF_CV
+        default: return GINF_ERR_UNSUPPORTED;
+    }
+    return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+int ginf_conv3x3_bwd_weight(const uint16_t *x, const uint16_t *dz, int N, int H, int W, int Cin, int Cout, float *dw,
+                            float *db, void *stream) {
+    if (!x || !dz || !dw || Cin % 16 || Cout % 16) return GINF_ERR_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned blocks = (unsigned)((Cout / 16) * (Cin / 16) * 9);
+    hipLaunchKernelGGL(k_conv3x3_dw, dim3(blocks), dim3(256), 0, st, x, dz, N, H, W, Cin, Cout, dw);
+    if (db) hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)Cout), dim3(256), 0, st, dz, (int64_t)N * H * W, Cout, db);
+    return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+int ginf_conv3x3_flip_weights(const uint16_t *w, int Cout, int Cin, uint16_t *wf, void *stream) {
+    if (!w || !wf) return GINF_ERR_INVALID;
+    hipLaunchKernelGGL(k_flip_weights, dim3(nblk((int64_t)Cout * 9 * Cin)), dim3(256), 0, (hipStream_t)stream, w,
+                       Cout, Cin, wf);
+    return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+int ginf_relu_bwd(const float *dy, const uint16_t *y, int64_t n, uint16_t *dz, void *stream) {
+    if (!dy || !dz) return GINF_ERR_INVALID;
+    hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, dy, y, n, dz);
+    return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+int ginf_mse(const float *o, const float *t, int64_t n, float *grad, float *partials /* >= 1024 */, int *nparts,
+             void *stream) {
+    if (!o || !t || !grad || !partials || !nparts) return GINF_ERR_INVALID;
+    const unsigned b = nblk(n) > 1024 ? 1024 : nblk(n);
+    hipLaunchKernelGGL(k_mse, dim3(b), dim3(256), 0, (hipStream_t)stream, o, t, n, grad, partials);
+    *nparts = (int)b;
+    return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+int ginf_adam(float *w, const float *g, float *m1, float *m2, uint16_t *wb, int64_t n, float lr, float b1, float b2,
+              float eps, int step, float wd, void *stream) {
+    if (!w || !g || !m1 || !m2 || step < 1) return GINF_ERR_INVALID;
+    const float c1 = 1.f - __builtin_powf(b1, (float)step), c2 = 1.f - __builtin_powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, w, g, m1, m2, wb, n, lr, b1, b2, eps,
+                       c1, c2, wd);
+    return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+int ginf_to_bf16(const float *s, int64_t n, uint16_t *d, void *stream) {
+    if (!s || !d) return GINF_ERR_INVALID;
+    hipLaunchKernelGGL(k_to_bf16, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, s, n, d);
+    return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+int ginf_minpool_costs(const void *costs, int dtype, int W, int H, int L, int f, float scale, uint16_t *out,
+                       void *stream) {
+    if (!costs || !out || f < 1) return GINF_ERR_INVALID;
+    const int w4 = (W + f - 1) / f, h4 = (H + f - 1) / f;
+    const int64_t n = (int64_t)w4 * h4 * L;
+    if (dtype == GINF_F32)
+        hipLaunchKernelGGL(k_minpool<float>, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, (const float *)costs, W,
+                           H, L, f, scale, w4, h4, out);
+    else
+        hipLaunchKernelGGL(k_minpool<int>, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, (const int *)costs, W, H,
+                           L, f, scale, w4, h4, out);
+    return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+int ginf_stereo_costs(const uint8_t *left, const uint8_t *right, int W, int H, int L, int trunc, float *out,
+                      void *stream) {
+    if (!left || !right || !out || W < 1 || H < 1 || L < 1) return GINF_ERR_INVALID;
+    hipLaunchKernelGGL(k_stereo_costs, dim3(nblk((int64_t)W * H * L)), dim3(256), 0, (hipStream_t)stream, left, right,
+                       W, H, L, trunc, out);
+    return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+int ginf_block_mean_msgs(const float *msgs, int W, int H, int L, int f, float scale, float *out, void *stream) {
+    if (!msgs || !out || f < 1) return GINF_ERR_INVALID;
+    const int w4 = (W + f - 1) / f, h4 = (H + f - 1) / f;
+    const int64_t n = (int64_t)w4 * h4 * 4 * L;
+    hipLaunchKernelGGL(k_block_mean_msgs, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, msgs, W, H, L, f, scale,
+                       w4, h4, out);
+    return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+}  // extern "C"
+
+// Used by bp_plan.cpp (plan internals stay there).
+namespace ginf {
+hipError_t launch_inject(const float *pred, void *const msgs[2][4], const LevelGeom &g, int C, int G, bool interleave,
+                         int L, int shift, int pw, int ph, float inv_scale, hipStream_t st) {
+    InjectArgs a{};
+    for (int c = 0; c < 2; ++c)
+        for (int d = 0; d < 4; ++d) a.msgs[c][d] = msgs[c][d];
+    a.g = g;
+    a.C = C;
+    a.G = G;
+    a.interleave = interleave ? 1 : 0;
+    a.L = L;
+    a.shift = shift;
+    a.pw = pw;
+    a.ph = ph;
+    a.inv_scale = inv_scale;
+    const int64_t n = (int64_t)g.w * g.h * 4 * L;
+    hipLaunchKernelGGL(k_inject, dim3(nblk(n)), dim3(256), 0, st, pred, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_mean_state(void *const msgs[2][4], const LevelGeom &g, int C, int G, bool interleave, int L,
+                                   int f, float scale, float *out, hipStream_t st) {
+    InjectArgs a{};
+    for (int c = 0; c < 2; ++c)
+        for (int d = 0; d < 4; ++d) a.msgs[c][d] = msgs[c][d];
+    a.g = g;
+    a.C = C;
+    a.G = G;
+    a.interleave = interleave ? 1 : 0;
+    a.L = L;
+    const int64_t n = (int64_t)((g.w + f - 1) / f) * ((g.h + f - 1) / f) * 4 * L;
+    hipLaunchKernelGGL(k_block_mean_state, dim3(nblk(n)), dim3(256), 0, st, a, f, scale, out);
+    return hipGetLastError();
+}
+}  // namespace ginf

@@ -109,11 +109,7 @@ __device__ __forceinline__ void quad_node(const SweepArgs &a, const int slot, co
                     else { f[l] = tmin(h, f[l - 1] + s); mu = tmin(mu, h); }
                 }
             }
-#ifdef GINF_DEBUG_NO_ARGMIN
-            if (false) {
-#else
             if (LABELS && (FULL || 4 * j + d < L)) {
-#endif
                 if (arg < 0 || bsel < best) { best = bsel; arg = 4 * j + d; }
             }
         }

@@ -1,0 +1,373 @@
+"""Learned warm start for hierarchical BP (project-owned; no reference
+counterpart).
+
+A 6-layer 3x3 convolutional network (BF16 weights and activations, FP32
+accumulation, 32 hidden channels, ReLU) maps the data-cost volume min-pooled
+to 1/4 resolution (L input channels) to the four directional message volumes
+at 1/4 resolution (4L output channels).  Its prediction seeds a fine level of
+the BP pyramid (outbox-frame upsample, border outboxes kept at zero), which
+then runs a few red-black half-sweeps; plain hierarchical BP stays the
+fallback and the judge of the energy.
+
+Every layer runs on the library's own HIP kernels (ginf_conv3x3_* on MFMA);
+PyTorch only provides device memory and the stream.  Training targets are the
+converged level-0 messages of plain HBP on seeded synthetic stereo
+instances, averaged over 4x4 blocks.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import ctypes
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib as G
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+MODELS = os.path.join(PKG, "models")
+
+POOL = 4          # input / output resolution factor
+IN_SCALE = 1.0 / 40.0   # cost scale (generator truncation 40)
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _ck(rc):
+    G.check(rc)
+
+
+class ConvNet:
+    """Weights [Cout][3][3][Cin] (FP32 master + BF16 copy), biases FP32, Adam state."""
+
+    def __init__(self, labels: int, hidden: int = 32, layers: int = 6, seed: int = 0):
+        import torch
+        self.torch = torch
+        G.lib()
+        self.L = labels
+        self.hidden = hidden
+        self.chans = [labels] + [hidden] * (layers - 1) + [4 * labels]
+        rng = np.random.default_rng(seed)
+        self.w, self.b, self.wb, self.wf = [], [], [], []
+        self.m1, self.m2 = [], []
+        for k in range(layers):
+            ci, co = self.chans[k], self.chans[k + 1]
+            std = np.sqrt(2.0 / (9 * ci)) * (0.5 if k == layers - 1 else 1.0)
+            w = torch.from_numpy((rng.standard_normal((co, 3, 3, ci)) * std).astype(np.float32)).cuda()
+            b = torch.zeros(co, dtype=torch.float32, device="cuda")
+            self.w.append(w)
+            self.b.append(b)
+            self.wb.append(torch.empty(w.shape, dtype=torch.int16, device="cuda"))
+            self.wf.append(torch.empty((ci, 3, 3, co), dtype=torch.int16, device="cuda"))
+            self.m1.append([torch.zeros_like(w), torch.zeros_like(b)])
+            self.m2.append([torch.zeros_like(w), torch.zeros_like(b)])
+        self.step_count = 0
+        self.refresh()
+        self._bufs = {}
+
+    @property
+    def layers(self):
+        return len(self.w)
+
+    def stream(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def refresh(self):
+        """BF16 weight copies and the flipped kernels used by the input gradient."""
+        for k in range(self.layers):
+            _ck(G.lib().ginf_to_bf16(_vp(self.w[k]), self.w[k].numel(), _vp(self.wb[k]), self.stream()))
+            _ck(G.lib().ginf_conv3x3_flip_weights(_vp(self.wb[k]), self.chans[k + 1], self.chans[k], _vp(self.wf[k]),
+                                                 self.stream()))
+
+    def params(self) -> int:
+        return int(sum(w.numel() + b.numel() for w, b in zip(self.w, self.b)))
+
+    def _buf(self, key, shape, dtype):
+        t = self._bufs.get(key)
+        if t is None or tuple(t.shape) != tuple(shape) or t.dtype != dtype:
+            t = self.torch.empty(shape, dtype=dtype, device="cuda")
+            self._bufs[key] = t
+        return t
+
+    def forward(self, x_bf16, N: int, H: int, W: int, out=None):
+        """x_bf16: int16 tensor (N,H,W,L) holding bf16.  Returns FP32 (N,H,W,4L);
+        keeps the layer inputs for backward."""
+        torch = self.torch
+        self.acts = [x_bf16]
+        h = x_bf16
+        for k in range(self.layers):
+            ci, co = self.chans[k], self.chans[k + 1]
+            last = k == self.layers - 1
+            if last:
+                y = out if out is not None else self._buf(("out",), (N, H, W, co), torch.float32)
+                _ck(G.lib().ginf_conv3x3_fwd(_vp(h), _vp(self.wb[k]), _vp(self.b[k]), N, H, W, ci, co, 0, None,
+                                            _vp(y), self.stream()))
+                self.out = y
+            else:
+                y = self._buf(("act", k), (N, H, W, co), torch.int16)
+                _ck(G.lib().ginf_conv3x3_fwd(_vp(h), _vp(self.wb[k]), _vp(self.b[k]), N, H, W, ci, co, 1, _vp(y),
+                                            None, self.stream()))
+                self.acts.append(y)
+                h = y
+        self.shape = (N, H, W)
+        return self.out
+
+    def backward(self, dout):
+        """dout: FP32 gradient of the loss w.r.t. the output.  Fills self.gw/self.gb."""
+        torch = self.torch
+        N, H, W = self.shape
+        self.gw = [None] * self.layers
+        self.gb = [None] * self.layers
+        dy = dout
+        for k in range(self.layers - 1, -1, -1):
+            ci, co = self.chans[k], self.chans[k + 1]
+            last = k == self.layers - 1
+            dz = self._buf(("dz", k), (N, H, W, co), torch.int16)
+            yk = None if last else self.acts[k + 1]
+            _ck(G.lib().ginf_relu_bwd(_vp(dy), _vp(yk) if yk is not None else None, dy.numel(), _vp(dz),
+                                     self.stream()))
+            gw = self._buf(("gw", k), tuple(self.w[k].shape), torch.float32)
+            gb = self._buf(("gb", k), (co,), torch.float32)
+            _ck(G.lib().ginf_conv3x3_bwd_weight(_vp(self.acts[k]), _vp(dz), N, H, W, ci, co, _vp(gw), _vp(gb),
+                                               self.stream()))
+            self.gw[k], self.gb[k] = gw, gb
+            if k > 0:
+                dx = self._buf(("dx", k), (N, H, W, ci), torch.float32)
+                _ck(G.lib().ginf_conv3x3_fwd(_vp(dz), _vp(self.wf[k]), None, N, H, W, co, ci, 0, None, _vp(dx),
+                                            self.stream()))
+                dy = dx
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        self.step_count += 1
+        for k in range(self.layers):
+            _ck(G.lib().ginf_adam(_vp(self.w[k]), _vp(self.gw[k]), _vp(self.m1[k][0]), _vp(self.m2[k][0]),
+                                 _vp(self.wb[k]), self.w[k].numel(), lr, b1, b2, eps, self.step_count, wd,
+                                 self.stream()))
+            _ck(G.lib().ginf_adam(_vp(self.b[k]), _vp(self.gb[k]), _vp(self.m1[k][1]), _vp(self.m2[k][1]), None,
+                                 self.b[k].numel(), lr, b1, b2, eps, self.step_count, 0.0, self.stream()))
+        for k in range(self.layers):
+            _ck(G.lib().ginf_conv3x3_flip_weights(_vp(self.wb[k]), self.chans[k + 1], self.chans[k], _vp(self.wf[k]),
+                                                 self.stream()))
+
+    def mse(self, target):
+        """Loss and its gradient (written into a buffer) for the last forward."""
+        torch = self.torch
+        grad = self._buf(("gout",), tuple(self.out.shape), torch.float32)
+        parts = self._buf(("parts",), (1024,), torch.float32)
+        nparts = ctypes.c_int()
+        _ck(G.lib().ginf_mse(_vp(self.out), _vp(target), self.out.numel(), _vp(grad), _vp(parts),
+                            ctypes.byref(nparts), self.stream()))
+        loss = parts[:nparts.value].sum() / self.out.numel()
+        return loss, grad
+
+    # ---- checkpoints ------------------------------------------------------
+    def state(self) -> dict:
+        d = {"L": self.L, "hidden": self.hidden, "layers": self.layers, "step": self.step_count}
+        arrs = {}
+        for k in range(self.layers):
+            arrs[f"w{k}"] = self.w[k].cpu().numpy()
+            arrs[f"b{k}"] = self.b[k].cpu().numpy()
+            arrs[f"m1w{k}"] = self.m1[k][0].cpu().numpy()
+            arrs[f"m1b{k}"] = self.m1[k][1].cpu().numpy()
+            arrs[f"m2w{k}"] = self.m2[k][0].cpu().numpy()
+            arrs[f"m2b{k}"] = self.m2[k][1].cpu().numpy()
+        return d, arrs
+
+    def save(self, path: str, extra: dict | None = None) -> None:
+        meta, arrs = self.state()
+        if extra:
+            meta.update(extra)
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        tmp = path + ".tmp.npz"
+        np.savez(tmp, meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8), **arrs)
+        os.replace(tmp, path)
+
+    @classmethod
+    def load(cls, path: str) -> "ConvNet":
+        import torch
+        z = np.load(path, allow_pickle=False)
+        meta = json.loads(bytes(z["meta"]).decode())
+        net = cls(meta["L"], meta["hidden"], meta["layers"])
+        for k in range(net.layers):
+            net.w[k].copy_(torch.from_numpy(z[f"w{k}"]))
+            net.b[k].copy_(torch.from_numpy(z[f"b{k}"]))
+            if f"m1w{k}" in z:
+                net.m1[k][0].copy_(torch.from_numpy(z[f"m1w{k}"]))
+                net.m1[k][1].copy_(torch.from_numpy(z[f"m1b{k}"]))
+                net.m2[k][0].copy_(torch.from_numpy(z[f"m2w{k}"]))
+                net.m2[k][1].copy_(torch.from_numpy(z[f"m2b{k}"]))
+        net.step_count = meta.get("step", 0)
+        net.meta = meta
+        net.refresh()
+        return net
+
+
+# ---------------------------------------------------------------- instances
+
+class Workload:
+    """A warm-start workload: grid size, labels, smoothness (truncated linear
+    lambda * min(|a-b|, tau) or Potts), generator parameters, BP schedule."""
+
+    def __init__(self, W=640, H=480, L=64, model=G.SM_TRUNC_LINEAR, lam=10, tau=3, rects=24, trunc=40,
+                 levels=5, iters=10):
+        self.W, self.H, self.L = W, H, L
+        self.model, self.lam, self.tau = model, lam, tau
+        self.rects, self.trunc = rects, trunc
+        self.levels, self.iters = levels, iters
+        a = np.arange(L)
+        if model == G.SM_POTTS:
+            self.table = (lam * (a[:, None] != a[None, :])).astype(np.float32)
+        else:
+            self.table = np.minimum(lam * np.abs(a[:, None] - a[None, :]), lam * tau).astype(np.float32)
+        self.msg_scale = 1.0 / float(max(1.0, lam * (tau if model != G.SM_POTTS else 1)))
+        self.w4, self.h4 = (W + POOL - 1) // POOL, (H + POOL - 1) // POOL
+
+    def key(self):
+        return f"{self.W}x{self.H}x{self.L}-{'potts' if self.model == G.SM_POTTS else 'tl'}{self.lam}-{self.tau}"
+
+
+def gen_images(seeds, wl: Workload, threads: int = 16):
+    """Left/right uint8 images of seeded instances (host generator, threaded)."""
+    def one(seed):
+        g = G.gen_stereo(seed, wl.W, wl.H, wl.L, wl.rects, wl.L - 1, wl.trunc, want=("left", "right"))
+        return g["left"], g["right"]
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, seeds))
+
+
+class Pipeline:
+    """Device-side instance preparation for one workload: costs from images,
+    plain HBP (the judge), targets and network inputs."""
+
+    def __init__(self, wl: Workload):
+        import torch
+        self.torch = torch
+        self.wl = wl
+        self.plan = G.Plan(G.make_desc(wl.W, wl.H, wl.L, G.F32, wl.model, wl.levels, wl.iters, G.NORM_MEAN),
+                           wl.table)
+        self.costs = torch.empty((wl.L, wl.H, wl.W), dtype=torch.float32, device="cuda")
+        self.labels = torch.zeros((wl.H, wl.W), dtype=torch.int32, device="cuda")
+        self.left = torch.empty((wl.H, wl.W), dtype=torch.uint8, device="cuda")
+        self.right = torch.empty((wl.H, wl.W), dtype=torch.uint8, device="cuda")
+        self.table_dev = torch.from_numpy(wl.table).cuda()
+
+    def stream(self):
+        return self.torch.cuda.current_stream().cuda_stream
+
+    def load_costs(self, left: np.ndarray, right: np.ndarray):
+        wl = self.wl
+        self.left.copy_(self.torch.from_numpy(left))
+        self.right.copy_(self.torch.from_numpy(right))
+        _ck(G.lib().ginf_stereo_costs(_vp(self.left), _vp(self.right), wl.W, wl.H, wl.L, wl.trunc, _vp(self.costs),
+                                     ctypes.c_void_p(self.stream())))
+
+    def net_input(self, out_slot):
+        wl = self.wl
+        _ck(G.lib().ginf_minpool_costs(_vp(self.costs), G.F32, wl.W, wl.H, wl.L, POOL, IN_SCALE, _vp(out_slot),
+                                      ctypes.c_void_p(self.stream())))
+
+    def plain(self):
+        self.plan.run(self.costs.data_ptr(), self.labels.data_ptr(), self.stream())
+
+    def target(self, out_slot):
+        self.plan.block_mean_msgs(0, POOL, self.wl.msg_scale, out_slot.data_ptr(), self.stream())
+
+    def energy(self) -> float:
+        wl = self.wl
+        return G.energy_f64(wl.W, wl.H, wl.L, G.F32, self.costs.data_ptr(), self.table_dev.data_ptr(),
+                            self.labels.data_ptr(), self.stream())
+
+
+# ---------------------------------------------------------------- training
+
+def train(wl: Workload, n_instances=20000, batch=16, lr=2e-3, reuse=2, seed_base=1_000_000, ckpt=None,
+          log=print, resume=None, time_budget_s=None):
+    """Online training: each batch of fresh seeded instances is labelled by
+    plain HBP on the GPU and used for `reuse` Adam steps."""
+    import torch
+    net = ConvNet.load(resume) if resume else ConvNet(wl.L)
+    pipe = Pipeline(wl)
+    xin = torch.empty((batch, wl.h4, wl.w4, wl.L), dtype=torch.int16, device="cuda")
+    tgt = torch.empty((batch, wl.h4, wl.w4, 4 * wl.L), dtype=torch.float32, device="cuda")
+    done = net.meta.get("instances", 0) if resume else 0
+    t0 = time.time()
+    history = []
+    pool = cf.ThreadPoolExecutor(max_workers=1)
+    seeds = list(range(seed_base + done, seed_base + n_instances))
+    nxt = pool.submit(gen_images, seeds[:batch], wl)
+    pos = 0
+    while pos < len(seeds):
+        imgs = nxt.result()
+        cur = seeds[pos:pos + batch]
+        pos += len(cur)
+        if pos < len(seeds):
+            nxt = pool.submit(gen_images, seeds[pos:pos + batch], wl)
+        nb = len(cur)
+        for k, (lft, rgt) in enumerate(imgs):
+            pipe.load_costs(lft, rgt)
+            pipe.net_input(xin[k])
+            pipe.plain()
+            pipe.target(tgt[k])
+        nbatches = max(1, len(seeds) // batch)
+        warm = min(20, nbatches // 5 + 1)
+        step_lr = lr * min(1.0, (pos / batch) / warm) * (0.1 ** (pos / max(1, len(seeds))))
+        for _ in range(reuse):
+            net.forward(xin[:nb], nb, wl.h4, wl.w4)
+            loss, g = net.mse(tgt[:nb])
+            net.backward(g)
+            net.adam(lr=step_lr)
+        done += nb
+        lv = float(loss)
+        history.append((done, lv))
+        if len(history) % 25 == 1 or pos >= len(seeds):
+            log(f"  instances {done:6d}  loss {lv:.5f}  lr {step_lr:.2e}  {time.time() - t0:.1f}s")
+        if ckpt and len(history) % 200 == 0:
+            net.save(ckpt, {"instances": done, "workload": wl.key()})
+        if time_budget_s and time.time() - t0 > time_budget_s:
+            log(f"  time budget reached after {done} instances")
+            break
+    if ckpt:
+        net.save(ckpt, {"instances": done, "workload": wl.key(), "loss": history[-1][1]})
+    return net, history
+
+
+# ---------------------------------------------------------------- inference
+
+class WarmStart:
+    """Network inference + seeded BP on one workload.  `seed_level` is the
+    pyramid level seeded from the 1/4-resolution prediction (0: finest, with
+    a 4x outbox upsample; 1: half resolution, 2x)."""
+
+    def __init__(self, wl: Workload, net: ConvNet | None):
+        import torch
+        self.torch = torch
+        self.wl = wl
+        self.net = net
+        self.plan = G.Plan(G.make_desc(wl.W, wl.H, wl.L, G.F32, wl.model, max(2, wl.levels), wl.iters, G.NORM_MEAN),
+                           wl.table)
+        self.xin = torch.empty((1, wl.h4, wl.w4, wl.L), dtype=torch.int16, device="cuda")
+        self.pred = torch.zeros((1, wl.h4, wl.w4, 4 * wl.L), dtype=torch.float32, device="cuda")
+
+    def stream(self):
+        return self.torch.cuda.current_stream().cuda_stream
+
+    def run(self, costs, labels, schedule, seed_level=1):
+        """schedule: half-sweeps per level for levels seed_level..0 (index = level)."""
+        wl = self.wl
+        if self.net is not None:
+            _ck(G.lib().ginf_minpool_costs(_vp(costs), G.F32, wl.W, wl.H, wl.L, POOL, IN_SCALE, _vp(self.xin),
+                                          ctypes.c_void_p(self.stream())))
+            self.net.forward(self.xin, 1, wl.h4, wl.w4, out=self.pred)
+        else:
+            self.pred.zero_()
+        shift = 2 - seed_level
+        self.plan.seed(seed_level, self.pred.data_ptr(), wl.w4, wl.h4, shift, wl.msg_scale, self.stream())
+        self.plan.run_schedule(costs.data_ptr(), labels.data_ptr(), seed_level, schedule, True, self.stream())
+
+
+def default_model_path(wl: Workload) -> str:
+    return os.path.join(MODELS, f"warmstart_{wl.key()}.npz")

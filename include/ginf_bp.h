@@ -134,6 +134,54 @@ int ginf_gen_stereo(uint64_t seed, int width, int height, int num_labels, int nu
                     int trunc, uint8_t *left, uint8_t *right, int32_t *disp, float *cost_f32,
                     int32_t *cost_i32);
 
+/* Scheduled run (no reference counterpart): like ginf_bp_plan_run, but
+ * iters[t] half-sweeps at level t for t <= start_level and levels above
+ * start_level skipped.  seeded != 0: level start_level begins from the
+ * outboxes already stored there by ginf_bp_plan_seed instead of zeros. */
+int ginf_bp_plan_run_schedule(ginf_bp_plan plan, const void *data_dev, int32_t *labels_dev, int start_level,
+                              const int *iters, int seeded, void *stream);
+
+/* Seed the outboxes of `level` from a device prediction pred_dev (FP32, NHWC
+ * [ph][pw][4L], channel d*L + l, d in S,E,N,W order) at 1/2^shift of the
+ * level's resolution: interior node (x, y) takes pred[y>>shift][x>>shift] /
+ * scale; border outboxes stay 0.  FP32 plans only. */
+int ginf_bp_plan_seed(ginf_bp_plan plan, int level, const float *pred_dev, int pw, int ph, int shift, float scale,
+                      void *stream);
+
+/* Learned warm start building blocks (no reference counterpart): BF16 3x3
+ * convolutions on MFMA with FP32 accumulation.  Tensors NHWC, weights
+ * [Cout][3][3][Cin], bf16 passed as uint16_t.  Cin % 32 == 0, Cout % 16 == 0. */
+int ginf_conv3x3_fwd(const uint16_t *x, const uint16_t *w, const float *bias, int N, int H, int W, int Cin, int Cout,
+                     int relu, uint16_t *y_bf16, float *y_f32, void *stream);
+/* dW [Cout][3][3][Cin] and db [Cout] from the layer input x and the
+ * pre-activation gradient dz (both bf16 NHWC); deterministic. */
+int ginf_conv3x3_bwd_weight(const uint16_t *x, const uint16_t *dz, int N, int H, int W, int Cin, int Cout, float *dw,
+                            float *db, void *stream);
+/* wf[ci][kh][kw][co] = w[co][2-kh][2-kw][ci]: forward convolution with wf is
+ * the input gradient (bwd-data) of w. */
+int ginf_conv3x3_flip_weights(const uint16_t *w, int Cout, int Cin, uint16_t *wf, void *stream);
+/* dz = bf16(dy * (y > 0)) (y = NULL: no ReLU). */
+int ginf_relu_bwd(const float *dy, const uint16_t *y, int64_t n, uint16_t *dz, void *stream);
+/* grad = 2 (o - t) / n; per-block squared-error partials (<= 1024). */
+int ginf_mse(const float *o, const float *t, int64_t n, float *grad, float *partials, int *nparts, void *stream);
+/* Adam step (bias-corrected, step >= 1) on FP32 weights; refreshes the bf16 copy wb when non-NULL. */
+int ginf_adam(float *w, const float *g, float *m1, float *m2, uint16_t *wb, int64_t n, float lr, float b1, float b2,
+              float eps, int step, float wd, void *stream);
+int ginf_to_bf16(const float *s, int64_t n, uint16_t *d, void *stream);
+/* Network input: costs [L][H][W] min-pooled over f x f blocks, times scale, bf16 NHWC. */
+int ginf_minpool_costs(const void *costs, int dtype, int W, int H, int L, int f, float scale, uint16_t *out,
+                       void *stream);
+/* Device stereo costs from an image pair (the cost rule of ginf_gen_stereo), FP32 [L][H][W]. */
+int ginf_stereo_costs(const uint8_t *left, const uint8_t *right, int W, int H, int L, int trunc, float *out,
+                      void *stream);
+/* Training target: outboxes [l][d][y][x] averaged over f x f blocks, times scale, FP32 NHWC [.][.][4L]. */
+int ginf_block_mean_msgs(const float *msgs, int W, int H, int L, int f, float scale, float *out, void *stream);
+
+/* Training target straight from a plan's state: outboxes of `level` after the
+ * last run averaged over f x f node blocks, times scale, FP32 NHWC
+ * [ceil(h/f)][ceil(w/f)][4L] with channel d*L + l (FP32 plans). */
+int ginf_bp_plan_block_mean_msgs(ginf_bp_plan plan, int level, int f, float scale, float *out_dev, void *stream);
+
 /* Batches (no reference counterpart; the batched workload of the benchmark
  * configurations): n independent single-level MRFs of any sizes and label
  * counts (Potts / truncated linear, L <= 128) sharing dtype, normalisation

@@ -1,0 +1,211 @@
+"""GPU tests of the learned warm start: the MFMA convolution kernels against
+the float64 oracle (forward, weight and input gradients, a whole network's
+gradients), the optimizer, the pooling / target / cost kernels, seeding and
+scheduled BP runs, and a short training run that lowers the loss."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from gpu_util import torch_cuda, say
+from gridinfcuda_amd import _lib as G
+from oracle import conv_oracle as C
+
+pytestmark = pytest.mark.gpu
+
+
+def dev_bf16(torch, a):
+    return torch.from_numpy(C.to_bf16_bits(a).view(np.int16)).cuda()
+
+
+def vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(1e-12, np.abs(b).max()))
+
+
+@pytest.mark.parametrize("N,H,W,Ci,Co", [(2, 7, 9, 32, 32), (1, 5, 33, 64, 32), (3, 4, 6, 32, 256), (1, 6, 7, 96, 384),
+                                         (2, 3, 17, 32, 64), (1, 9, 5, 256, 32)])
+def test_conv_forward_and_gradients(N, H, W, Ci, Co):
+    torch = torch_cuda()
+    say(f"conv N={N} H={H} W={W} Ci={Ci} Co={Co}")
+    rng = np.random.default_rng(Ci * 7 + Co)
+    x = C.bf16_round(rng.standard_normal((N, H, W, Ci)))
+    w = C.bf16_round(rng.standard_normal((Co, 3, 3, Ci)) * 0.2)
+    b = rng.standard_normal(Co).astype(np.float32)
+    xd, wd = dev_bf16(torch, x), dev_bf16(torch, w)
+    bd = torch.from_numpy(b).cuda()
+    yf = torch.empty((N, H, W, Co), dtype=torch.float32, device="cuda")
+    yb = torch.empty((N, H, W, Co), dtype=torch.int16, device="cuda")
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    G.check(G.lib().ginf_conv3x3_fwd(vp(xd), vp(wd), vp(bd), N, H, W, Ci, Co, 0, None, vp(yf), s))
+    G.check(G.lib().ginf_conv3x3_fwd(vp(xd), vp(wd), vp(bd), N, H, W, Ci, Co, 1, vp(yb), None, s))
+    torch.cuda.synchronize()
+    ref = C.conv3x3(x, w, b.astype(np.float64))
+    assert rel(yf.cpu().numpy(), ref) < 1e-5
+    relu_b = C.from_bf16_bits(yb.cpu().numpy().view(np.uint16))
+    np.testing.assert_array_equal(relu_b, C.bf16_round(np.maximum(yf.cpu().numpy(), 0)))
+    # gradients for dz
+    dz = C.bf16_round(rng.standard_normal((N, H, W, Co)))
+    dzd = dev_bf16(torch, dz)
+    dw = torch.empty((Co, 3, 3, Ci), dtype=torch.float32, device="cuda")
+    db = torch.empty(Co, dtype=torch.float32, device="cuda")
+    G.check(G.lib().ginf_conv3x3_bwd_weight(vp(xd), vp(dzd), N, H, W, Ci, Co, vp(dw), vp(db), s))
+    wf = torch.empty((Ci, 3, 3, Co), dtype=torch.int16, device="cuda")
+    G.check(G.lib().ginf_conv3x3_flip_weights(vp(wd), Co, Ci, vp(wf), s))
+    dx = torch.empty((N, H, W, Ci), dtype=torch.float32, device="cuda")
+    if Co % 32 == 0 and Ci % 16 == 0:
+        G.check(G.lib().ginf_conv3x3_fwd(vp(dzd), vp(wf), None, N, H, W, Co, Ci, 0, None, vp(dx), s))
+    torch.cuda.synchronize()
+    rdx, rdw, rdb = C.conv3x3_bwd(x, w, dz)
+    assert rel(dw.cpu().numpy(), rdw) < 1e-5
+    assert rel(db.cpu().numpy(), rdb) < 1e-5
+    if Co % 32 == 0:
+        assert rel(dx.cpu().numpy(), rdx) < 1e-5
+
+
+def test_network_gradients_match_oracle():
+    """Six-layer net (L=32 -> 32 x5 -> 128) forward + backward on the GPU vs the
+    oracle with the same bf16 rounding points."""
+    torch = torch_cuda()
+    from gridinfcuda_amd.warmstart import ConvNet
+    say("network gradient check")
+    net = ConvNet(32, seed=3)
+    rng = np.random.default_rng(4)
+    N, H, W = 2, 6, 8
+    x = C.bf16_round(rng.standard_normal((N, H, W, 32)))
+    t = rng.standard_normal((N, H, W, 128)).astype(np.float32)
+    out = net.forward(dev_bf16(torch, x), N, H, W)
+    loss, g = net.mse(torch.from_numpy(t).cuda())
+    net.backward(g)
+    torch.cuda.synchronize()
+    ws = [w.cpu().numpy().astype(np.float64) for w in net.w]
+    bs = [b.cpu().numpy().astype(np.float64) for b in net.b]
+    rout, ins, outs = C.net_forward(x, ws, bs)
+    assert rel(out.cpu().numpy(), rout) < 1e-3
+    n = rout.size
+    rloss = float(((rout - t) ** 2).sum() / n)
+    assert abs(float(loss) - rloss) <= 1e-3 * rloss
+    grads, _ = C.net_backward(2 * (rout - t) / n, ws, ins, outs)
+    for k in range(6):
+        gw = net.gw[k].cpu().numpy()
+        gb = net.gb[k].cpu().numpy()
+        err_w = np.linalg.norm(gw - grads[k][0]) / max(1e-30, np.linalg.norm(grads[k][0]))
+        err_b = np.linalg.norm(gb - grads[k][1]) / max(1e-30, np.linalg.norm(grads[k][1]))
+        say(f"  layer {k}: rel grad err w {err_w:.2e} b {err_b:.2e}")
+        assert err_w < 2e-2 and err_b < 2e-2
+
+
+def test_adam_matches_numpy():
+    torch = torch_cuda()
+    rng = np.random.default_rng(5)
+    n = 1000
+    w = rng.standard_normal(n).astype(np.float32)
+    g = rng.standard_normal(n).astype(np.float32)
+    m1 = np.zeros(n, np.float32)
+    m2 = np.zeros(n, np.float32)
+    wd, gd = torch.from_numpy(w.copy()).cuda(), torch.from_numpy(g).cuda()
+    m1d, m2d = torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda")
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for step in (1, 2, 3):
+        G.check(G.lib().ginf_adam(vp(wd), vp(gd), vp(m1d), vp(m2d), None, n, 1e-2, 0.9, 0.999, 1e-8, step, 0.0, s))
+        m1 = 0.9 * m1 + 0.1 * g
+        m2 = 0.999 * m2 + 0.001 * g * g
+        w = w - 1e-2 * (m1 / (1 - 0.9 ** step)) / (np.sqrt(m2 / (1 - 0.999 ** step)) + 1e-8)
+    np.testing.assert_allclose(wd.cpu().numpy(), w, rtol=1e-5, atol=1e-6)
+
+
+def test_costs_pool_and_targets():
+    torch = torch_cuda()
+    from gridinfcuda_amd.warmstart import Workload, Pipeline, POOL, IN_SCALE
+    say("costs / minpool / block-mean")
+    wl = Workload(W=70, H=45, L=32, rects=6, levels=3, iters=4)
+    pipe = Pipeline(wl)
+    g = G.gen_stereo(11, wl.W, wl.H, wl.L, wl.rects, wl.L - 1, wl.trunc, want=("left", "right", "cost_f32"))
+    pipe.load_costs(g["left"], g["right"])
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(pipe.costs.cpu().numpy(), g["cost_f32"])
+    xin = torch.empty((wl.h4, wl.w4, wl.L), dtype=torch.int16, device="cuda")
+    pipe.net_input(xin)
+    c = g["cost_f32"]
+    ref = np.full((wl.L, wl.h4, wl.w4), np.inf)
+    for y in range(wl.H):
+        for x in range(wl.W):
+            ref[:, y // POOL, x // POOL] = np.minimum(ref[:, y // POOL, x // POOL], c[:, y, x])
+    got = C.from_bf16_bits(xin.cpu().numpy().view(np.uint16))
+    np.testing.assert_array_equal(got, C.bf16_round(np.transpose(ref, (1, 2, 0)) * IN_SCALE))
+    pipe.plain()
+    tgt = torch.empty((wl.h4, wl.w4, 4 * wl.L), dtype=torch.float32, device="cuda")
+    pipe.target(tgt)
+    torch.cuda.synchronize()
+    msgs = pipe.plan.dump_msgs(0)  # [l][d][y][x]
+    refm = np.zeros((wl.h4, wl.w4, 4 * wl.L))
+    for Y in range(wl.h4):
+        for X in range(wl.w4):
+            blk = msgs[:, :, Y * POOL:(Y + 1) * POOL, X * POOL:(X + 1) * POOL].mean(axis=(2, 3))  # [l][d]
+            refm[Y, X] = blk.T.reshape(-1) * wl.msg_scale
+    np.testing.assert_allclose(tgt.cpu().numpy(), refm, rtol=1e-5, atol=1e-6)
+
+
+def test_seed_and_schedule(oracle_mod):
+    torch = torch_cuda()
+    say("seed / scheduled runs")
+    L, W, H = 16, 40, 30
+    data = G.gen_stereo(3, W, H, L, 6, 15, 20)["cost_f32"]
+    table = oracle_mod.table_trunc_linear(L, 4, 12)
+    plan = G.Plan(G.make_desc(W, H, L, G.F32, G.SM_TRUNC_LINEAR, 3, 5), table)
+    dd = torch.from_numpy(data).cuda()
+    lab = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    # unseeded schedule = plain run
+    plan.run(dd.data_ptr(), lab.data_ptr(), s)
+    torch.cuda.synchronize()
+    ref = lab.cpu().numpy().copy()
+    lab.zero_()
+    plan.run_schedule(dd.data_ptr(), lab.data_ptr(), 2, [5, 5, 5], False, s)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(lab.cpu().numpy(), ref)
+    # a zero seed at the top level is the same as starting from zero messages
+    pw, ph = (W + 3) // 4, (H + 3) // 4
+    pred = torch.zeros((ph, pw, 4 * L), dtype=torch.float32, device="cuda")
+    plan.seed(2, pred.data_ptr(), pw, ph, 0, 1.0, s)
+    lab.zero_()
+    plan.run_schedule(dd.data_ptr(), lab.data_ptr(), 2, [5, 5, 5], True, s)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(lab.cpu().numpy(), ref)
+    # seeded level: interior outboxes = pred / scale, borders 0
+    rng = np.random.default_rng(6)
+    p = rng.standard_normal((ph, pw, 4 * L)).astype(np.float32)
+    pred.copy_(torch.from_numpy(p))
+    plan.seed(0, pred.data_ptr(), pw, ph, 2, 0.5, s)
+    m = plan.dump_msgs(0)
+    for (x, y) in [(1, 1), (W - 2, H - 2), (17, 9), (0, 5), (W - 1, 3)]:
+        for d in range(4):
+            want = np.zeros(L) if x in (0, W - 1) or y in (0, H - 1) else p[y >> 2, x >> 2, d * L:(d + 1) * L] / 0.5
+            np.testing.assert_allclose(m[:, d, y, x], want, rtol=1e-6)
+    # seeded level-0 run with k half-sweeps equals the oracle started from those messages
+    lab.zero_()
+    plan.run_schedule(dd.data_ptr(), lab.data_ptr(), 0, [3], True, s)
+    torch.cuda.synchronize()
+    msgs0 = m.copy()
+    for (x, y) in [(x, y) for y in range(H) for x in range(W) if x in (0, W - 1) or y in (0, H - 1)]:
+        msgs0[:, :, y, x] = 0
+    bel = np.zeros_like(data)
+    for it in range(3):
+        oracle_mod.half_sweep(data, msgs0, bel, table, 2, 0, it)
+    want = oracle_mod.labels_from_belief(bel, np.zeros((H, W), np.int32))
+    np.testing.assert_array_equal(lab.cpu().numpy()[1:-1, 1:-1], want[1:-1, 1:-1])
+    np.testing.assert_array_equal(plan.dump_msgs(0), msgs0)
+
+
+def test_training_lowers_loss():
+    torch = torch_cuda()
+    from gridinfcuda_amd.warmstart import Workload, train
+    say("short training run")
+    wl = Workload(W=96, H=64, L=32, rects=8, levels=3, iters=6)
+    net, hist = train(wl, n_instances=96, batch=16, lr=3e-3, reuse=4, seed_base=50, log=say)
+    first, last = hist[0][1], hist[-1][1]
+    say(f"  loss {first:.4f} -> {last:.4f}")
+    assert last < 0.7 * first
