@@ -356,8 +356,13 @@ class WarmStart:
         return self.torch.cuda.current_stream().cuda_stream
 
     def run(self, costs, labels, schedule, seed_level=1):
-        """schedule: half-sweeps per level for levels seed_level..0 (index = level)."""
+        """schedule: half-sweeps per level for levels seed_level..0 (index =
+        level).  seed_level < 0: plain HBP on all levels, no network."""
         wl = self.wl
+        if seed_level < 0:
+            self.plan.run_schedule(costs.data_ptr(), labels.data_ptr(), len(schedule) - 1, schedule, False,
+                                   self.stream())
+            return
         if self.net is not None:
             _ck(G.lib().ginf_minpool_costs(_vp(costs), G.F32, wl.W, wl.H, wl.L, POOL, IN_SCALE, _vp(self.xin),
                                           ctypes.c_void_p(self.stream())))
@@ -371,3 +376,119 @@ class WarmStart:
 
 def default_model_path(wl: Workload) -> str:
     return os.path.join(MODELS, f"warmstart_{wl.key()}.npz")
+
+
+# ---------------------------------------------------------------- evaluation
+
+def _timed(torch, fn, reps=5):
+    st = torch.cuda.current_stream()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        fn()
+        b.record(st)
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    return float(np.median(ts))
+
+
+# (seed level, half-sweeps per level from the seeded level down to level 0);
+# seed level -1 is plain HBP from zero messages on all 5 levels (a baseline
+# with fewer iterations than the judge's 10 per level).
+SCHEDULES = [(0, (4,)), (0, (6,)), (0, (8,)), (0, (10,)), (0, (14,)),
+             (1, (2, 4)), (1, (4, 4)), (1, (4, 6)), (1, (6, 6)), (1, (6, 8)), (1, (10, 10)),
+             (2, (4, 4, 6)), (2, (6, 6, 6)), (2, (6, 6, 8)), (2, (10, 10, 10)),
+             (-1, (4, 4, 4, 4, 4)), (-1, (6, 6, 6, 6, 6)), (-1, (10, 10, 10, 6, 6)), (-1, (10, 10, 10, 10, 8))]
+
+
+def evaluate(wl: Workload, net: ConvNet | None, seeds, schedules=SCHEDULES, reps=5, log=print):
+    """Energy and GPU time of plain HBP (the judge) against seeded schedules,
+    per instance.  A schedule (level, iters) seeds `level` from the network
+    (or from zeros when net is None) and runs iters[t] half-sweeps at level
+    level-t ... 0."""
+    import torch
+    pipe = Pipeline(wl)
+    ws = WarmStart(wl, net)
+    lab = torch.zeros((wl.H, wl.W), dtype=torch.int32, device="cuda")
+    imgs = gen_images(list(seeds), wl)
+    out = []
+    for seed, (lft, rgt) in zip(seeds, imgs):
+        pipe.load_costs(lft, rgt)
+        t_plain = _timed(torch, pipe.plain, reps)
+        e_plain = pipe.energy()
+        rec = {"seed": seed, "plain_ms": t_plain, "plain_energy": e_plain, "warm": []}
+        for level, iters in schedules:
+            sched = list(reversed(iters))
+
+            def go():
+                ws.run(pipe.costs, lab, sched, seed_level=level)
+            t = _timed(torch, go, reps)
+            e = G.energy_f64(wl.W, wl.H, wl.L, G.F32, pipe.costs.data_ptr(), pipe.table_dev.data_ptr(),
+                             lab.data_ptr(), torch.cuda.current_stream().cuda_stream)
+            rec["warm"].append({"level": level, "iters": list(iters), "ms": t, "energy": e,
+                                "rel_excess": e / e_plain - 1.0})
+        out.append(rec)
+        best = min((w for w in rec["warm"] if w["rel_excess"] <= 1e-3), key=lambda w: w["ms"], default=None)
+        log(f"  seed {seed}: plain {t_plain:.3f} ms E={e_plain:.0f}; fastest within 0.1%: "
+            + (f"{best['level']}/{best['iters']} {best['ms']:.3f} ms ({best['ms'] / t_plain:.2f}x)" if best else "none"))
+    return out
+
+
+def summarize(records, schedules=SCHEDULES):
+    """Per schedule: mean time ratio to plain, fraction of instances within 0.1 %."""
+    rows = []
+    for k, (level, iters) in enumerate(schedules):
+        ratios = [r["warm"][k]["ms"] / r["plain_ms"] for r in records]
+        ok = [r["warm"][k]["rel_excess"] <= 1e-3 for r in records]
+        exc = [r["warm"][k]["rel_excess"] for r in records]
+        rows.append({"level": level, "iters": list(iters), "time_ratio": float(np.mean(ratios)),
+                     "within_0.1pct": float(np.mean(ok)), "mean_rel_excess": float(np.mean(exc))})
+    return rows
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="train / evaluate the learned warm start")
+    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("--W", type=int, default=640)
+    ap.add_argument("--H", type=int, default=480)
+    ap.add_argument("--L", type=int, default=64)
+    ap.add_argument("--potts", action="store_true")
+    ap.add_argument("--lam", type=float, default=10)
+    ap.add_argument("--tau", type=float, default=3)
+    ap.add_argument("--instances", type=int, default=20000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--reuse", type=int, default=2)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--model", default=None)
+    ap.add_argument("--resume", default=None)
+    ap.add_argument("--budget", type=float, default=None, help="training time budget in seconds")
+    ap.add_argument("--eval-seeds", type=int, default=8)
+    ap.add_argument("--out", default=None, help="eval: write JSON records here")
+    a = ap.parse_args(argv)
+    wl = Workload(a.W, a.H, a.L, G.SM_POTTS if a.potts else G.SM_TRUNC_LINEAR, a.lam, a.tau)
+    path = a.model or default_model_path(wl)
+    if a.cmd == "train":
+        net, hist = train(wl, a.instances, a.batch, a.lr, a.reuse, ckpt=path, resume=a.resume, time_budget_s=a.budget)
+        print(json.dumps({"model": path, "params": net.params(), "instances": hist[-1][0], "final_loss": hist[-1][1],
+                          "first_loss": hist[0][1]}))
+    else:
+        net = ConvNet.load(path)
+        seeds = list(range(2_000_000, 2_000_000 + a.eval_seeds))
+        rec_net = evaluate(wl, net, seeds)
+        rec_zero = evaluate(wl, None, seeds)
+        res = {"workload": wl.key(), "model": path, "with_net": summarize(rec_net), "zero_seed": summarize(rec_zero),
+               "records": rec_net}
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f, indent=1)
+        for name in ("with_net", "zero_seed"):
+            print(name)
+            for r in res[name]:
+                print(f"  seed level {r['level']} iters {r['iters']}: time {r['time_ratio']:.2f}x plain, "
+                      f"within 0.1%: {r['within_0.1pct']:.2f}, mean excess {r['mean_rel_excess']:.2e}")
+
+
+if __name__ == "__main__":
+    main()
