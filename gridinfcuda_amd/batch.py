@@ -28,8 +28,10 @@ def _declare():
     L.ginf_batch_updates_per_run.argtypes = [VP, ctypes.POINTER(ctypes.c_double)]
     L.ginf_batch_dump_msgs.argtypes = [VP, I, VP]
     L.ginf_batch_destroy.argtypes = [VP]
+    L.ginf_batch_set_seeded.argtypes = [VP, I]
+    L.ginf_batch_seed.argtypes = [VP, I, VP, I, I, I, ctypes.c_float, VP]
     for n in ("ginf_batch_create", "ginf_batch_run", "ginf_batch_set_tiles", "ginf_batch_updates_per_run",
-              "ginf_batch_dump_msgs", "ginf_batch_destroy"):
+              "ginf_batch_dump_msgs", "ginf_batch_destroy", "ginf_batch_set_seeded", "ginf_batch_seed"):
         getattr(L, n).restype = I
     _declared = True
 
@@ -58,6 +60,13 @@ class Batch:
         dp = (ctypes.c_void_p * self.n)(*data_ptrs)
         lp = (ctypes.c_void_p * self.n)(*label_ptrs)
         G.check(G.lib().ginf_batch_run(self._h, dp, lp, ctypes.c_void_p(stream)))
+
+    def set_seeded(self, seeded: bool) -> None:
+        G.check(G.lib().ginf_batch_set_seeded(self._h, 1 if seeded else 0))
+
+    def seed(self, i: int, pred_ptr: int, pw: int, ph: int, shift: int, scale: float, stream: int = 0) -> None:
+        G.check(G.lib().ginf_batch_seed(self._h, i, ctypes.c_void_p(pred_ptr), pw, ph, shift, scale,
+                                        ctypes.c_void_p(stream)))
 
     def updates_per_run(self) -> float:
         u = ctypes.c_double()

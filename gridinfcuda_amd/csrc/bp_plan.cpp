@@ -599,6 +599,7 @@ struct ginf_batch_s {
     int slots_per_item[4] = {256, 256, 256, 256};  // by label capacity 16, 32, 64, 128
     int blocks_per_cu = 4;
     int num_cu = 0;
+    bool seeded = false;  // first half-sweep reads the outboxes stored by ginf_batch_seed
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -669,7 +670,7 @@ int batch_capture(ginf_batch b, const void *const *data_dev, int32_t *const *lab
                                            p->lv[0].data[0], p->lv[0].data[1], st);
         if (e != hipSuccess) return fail(e);
         if (N <= 1) {
-            if (N == 1) {
+            if (N == 1 && !b->seeded) {
                 const int64_t per = (int64_t)ly.C * p->geom[0].nplane * ly.G;
                 for (int c = 0; c < 2; ++c)
                     for (int k = 0; k < 4; ++k)
@@ -683,7 +684,7 @@ int batch_capture(ginf_batch b, const void *const *data_dev, int32_t *const *lab
     const int grid = b->num_cu * b->blocks_per_cu;
     for (int it = 0; it < N; ++it) {
         const int color = (it + 1) & 1;
-        const bool zeroin = N >= 2 && it == 0;
+        const bool zeroin = N >= 2 && it == 0 && !b->seeded;
         const bool labels = it >= N - 2;
         hipError_t e = launch_sweep_batch(b->dtype, b->args_dev + (size_t)color * b->n, b->items_dev, b->nitems, grid,
                                           zeroin, labels, st);
@@ -797,6 +798,19 @@ int ginf_batch_run(ginf_batch b, const void *const *data_dev, int32_t *const *la
     }
     GINF_HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));
     return GINF_OK;
+}
+
+int ginf_batch_set_seeded(ginf_batch b, int seeded) {
+    if (!b) return set_error(GINF_ERR_INVALID, "NULL batch");
+    if (b->seeded != (seeded != 0)) batch_drop_graph(b);
+    b->seeded = seeded != 0;
+    return GINF_OK;
+}
+
+int ginf_batch_seed(ginf_batch b, int instance, const float *pred_dev, int pw, int ph, int shift, float scale,
+                    void *stream) {
+    if (!b || instance < 0 || instance >= b->n) return set_error(GINF_ERR_INVALID, "bad instance");
+    return ginf_bp_plan_seed(b->plans[instance], 0, pred_dev, pw, ph, shift, scale, stream);
 }
 
 int ginf_batch_updates_per_run(ginf_batch b, double *updates) {

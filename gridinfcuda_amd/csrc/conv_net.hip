@@ -34,53 +34,54 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
 }
 
-// y = conv3x3(x, w) + bias (zero padding 1), optional ReLU; each wave owns
-// 16 consecutive pixels (flattened n,h,w) and all Cout = 16*CT channels.
-template <int CT>
+// y = conv3x3(x, w) + bias (zero padding 1), optional ReLU.  Each wave owns
+// 16 consecutive pixels (flattened n,h,w) and 32 output channels (two MFMA
+// column tiles); blockIdx.y selects the 32-channel group, so a wide output
+// layer spreads over more waves.
 __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
-                                                 const float *__restrict__ bias, int N, int H, int W, int Cin, int relu,
-                                                 uint16_t *__restrict__ yb, float *__restrict__ yf) {
+                                                 const float *__restrict__ bias, int N, int H, int W, int Cin,
+                                                 int Cout, int relu, uint16_t *__restrict__ yb,
+                                                 float *__restrict__ yf) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int64_t P = (int64_t)N * H * W;
     const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
     if (p0 >= P) return;
+    const int cg = blockIdx.y;  // output channels 32*cg .. 32*cg+31
     const int m = lane & 15, kg = lane >> 4;
     const int64_t p = p0 + m;
     const bool valid = p < P;
     const int n = valid ? (int)(p / ((int64_t)H * W)) : 0;
     const int rem = valid ? (int)(p - (int64_t)n * H * W) : 0;
     const int h = rem / W, wc = rem - (rem / W) * W;
-    const int Cout = 16 * CT;
-    f4 acc[CT];
-#pragma unroll
-    for (int t = 0; t < CT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    f4 acc0 = f4{0.f, 0.f, 0.f, 0.f}, acc1 = f4{0.f, 0.f, 0.f, 0.f};
     const bf16x8 zero = __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
+    const int64_t wstride = (int64_t)16 * 9 * Cin;  // one 16-channel tile of weights
     for (int kh = 0; kh < 3; ++kh) {
         for (int kw = 0; kw < 3; ++kw) {
             const int hh = h + kh - 1, ww = wc + kw - 1;
             const bool inb = valid && hh >= 0 && hh < H && ww >= 0 && ww < W;
             const uint16_t *xp = x + (((int64_t)n * H + (inb ? hh : 0)) * W + (inb ? ww : 0)) * Cin + kg * 8;
-            const uint16_t *wp = w + ((int64_t)m * 9 + kh * 3 + kw) * Cin + kg * 8;
+            const uint16_t *wp = w + ((int64_t)(cg * 32 + m) * 9 + kh * 3 + kw) * Cin + kg * 8;
             for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
                 const bf16x8 a = inb ? load8(xp + ci0) : zero;
-#pragma unroll
-                for (int t = 0; t < CT; ++t) {
-                    const bf16x8 b = load8(wp + (int64_t)t * 16 * 9 * Cin + ci0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
-                }
+                const bf16x8 b0 = load8(wp + ci0);
+                const bf16x8 b1 = load8(wp + wstride + ci0);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc1, 0, 0, 0);
             }
         }
     }
 #pragma unroll
-    for (int t = 0; t < CT; ++t) {
-        const int co = t * 16 + m;
+    for (int t = 0; t < 2; ++t) {
+        const int co = cg * 32 + t * 16 + m;
         const float bv = bias ? bias[co] : 0.f;
+        const f4 acc = t ? acc1 : acc0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int64_t pp = p0 + kg * 4 + i;
             if (pp >= P) continue;
-            float v = acc[t][i] + bv;
+            float v = acc[i] + bv;
             if (relu) v = v > 0.f ? v : 0.f;
             if (yb) yb[pp * Cout + co] = f2bf(v);
             if (yf) yf[pp * Cout + co] = v;
@@ -345,18 +346,12 @@ extern "C" {
 
 int ginf_conv3x3_fwd(const uint16_t *x, const uint16_t *w, const float *bias, int N, int H, int W, int Cin, int Cout,
                      int relu, uint16_t *y_bf16, float *y_f32, void *stream) {
-    if (!x || !w || (!y_bf16 && !y_f32) || N < 1 || H < 1 || W < 1 || Cin % 32 || Cout % 16 || Cout < 16 ||
-        Cout > 512)
+    if (!x || !w || (!y_bf16 && !y_f32) || N < 1 || H < 1 || W < 1 || Cin % 32 || Cout % 32 || Cout < 32)
         return GINF_ERR_INVALID;
     const int64_t P = (int64_t)N * H * W;
-    const dim3 grid((unsigned)((P + 63) / 64)), block(256);
-    hipStream_t st = (hipStream_t)stream;
-    switch (Cout / 16) {
-#define GINF_CV(CT) case CT: hipLaunchKernelGGL((k_conv3x3<CT>), grid, block, 0, st, x, w, bias, N, H, W, Cin, relu, y_bf16, y_f32); break;
-        GINF_CV(1) GINF_CV(2) GINF_CV(4) GINF_CV(6) GINF_CV(8) GINF_CV(12) GINF_CV(16) GINF_CV(24) GINF_CV(32)
-#undef GINF_CV
-        default: return GINF_ERR_UNSUPPORTED;
-    }
+    const dim3 grid((unsigned)((P + 63) / 64), (unsigned)(Cout / 32)), block(256);
+    hipLaunchKernelGGL(k_conv3x3, grid, block, 0, (hipStream_t)stream, x, w, bias, N, H, W, Cin, Cout, relu, y_bf16,
+                       y_f32);
     return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
 }
 

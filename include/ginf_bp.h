@@ -150,7 +150,7 @@ int ginf_bp_plan_seed(ginf_bp_plan plan, int level, const float *pred_dev, int p
 
 /* Learned warm start building blocks (no reference counterpart): BF16 3x3
  * convolutions on MFMA with FP32 accumulation.  Tensors NHWC, weights
- * [Cout][3][3][Cin], bf16 passed as uint16_t.  Cin % 32 == 0, Cout % 16 == 0. */
+ * [Cout][3][3][Cin], bf16 passed as uint16_t.  Cin % 32 == 0, Cout % 32 == 0. */
 int ginf_conv3x3_fwd(const uint16_t *x, const uint16_t *w, const float *bias, int N, int H, int W, int Cin, int Cout,
                      int relu, uint16_t *y_bf16, float *y_f32, void *stream);
 /* dW [Cout][3][3][Cin] and db [Cout] from the layer input x and the
@@ -196,6 +196,12 @@ int ginf_batch_run(ginf_batch batch, const void *const *data_dev, int32_t *const
  * (multiples of 64) and persistent blocks per compute unit. */
 int ginf_batch_set_tiles(ginf_batch batch, const int *slots_per_item, int blocks_per_cu);
 int ginf_batch_updates_per_run(ginf_batch batch, double *node_label_updates);
+/* Seeded batches: with seeded != 0 every instance starts from the outboxes
+ * stored by ginf_batch_seed (ginf_bp_plan_seed on the instance) instead of
+ * zeros. */
+int ginf_batch_set_seeded(ginf_batch batch, int seeded);
+int ginf_batch_seed(ginf_batch batch, int instance, const float *pred_dev, int pw, int ph, int shift, float scale,
+                    void *stream);
 int ginf_batch_dump_msgs(ginf_batch batch, int instance, void *msgs_host);
 int ginf_batch_destroy(ginf_batch batch);
 

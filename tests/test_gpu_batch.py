@@ -83,3 +83,46 @@ def test_batch_rejects_bad_instances():
     d = [G.make_desc(10, 10, 200)]
     with pytest.raises(G.GinfError):
         B.Batch(d, [B.table_potts(200, 1)])
+
+
+def test_seeded_batch(oracle_mod):
+    """A zero seed reproduces the unseeded batch bit for bit; a random seed
+    runs every instance from exactly the injected outboxes."""
+    torch = torch_cuda()
+    say("seeded batch")
+    insts = [x for x in small_instances(10, 31) if x["W"] >= 4 and x["H"] >= 4]
+    datas = [B.instance_costs(x) for x in insts]
+    iters = 4
+    bt = B.make_batch(insts, iters=iters)
+    inits = [np.zeros((x["H"], x["W"]), np.int32) for x in insts]
+    _, ll_ref = run_batch(torch, bt, datas, inits)
+    ref_msgs = [bt.dump_msgs(k) for k in range(len(insts))]
+    s = torch.cuda.current_stream().cuda_stream
+    preds = []
+    for x in insts:
+        pw, ph = (x["W"] + 3) // 4, (x["H"] + 3) // 4
+        preds.append(torch.zeros((ph, pw, 4 * x["L"]), dtype=torch.float32, device="cuda"))
+    bt.set_seeded(True)
+    for k, x in enumerate(insts):
+        bt.seed(k, preds[k].data_ptr(), preds[k].shape[1], preds[k].shape[0], 0, 1.0, s)
+    _, ll = run_batch(torch, bt, datas, inits)
+    for k in range(len(insts)):
+        np.testing.assert_array_equal(ll[k].cpu().numpy(), ll_ref[k].cpu().numpy(), err_msg=f"zero seed inst {k}")
+        np.testing.assert_array_equal(bt.dump_msgs(k), ref_msgs[k], err_msg=f"zero seed inst {k}")
+    rng = np.random.default_rng(8)
+    for k, x in enumerate(insts):
+        preds[k].copy_(torch.from_numpy(rng.standard_normal(tuple(preds[k].shape)).astype(np.float32) * 10))
+        bt.seed(k, preds[k].data_ptr(), preds[k].shape[1], preds[k].shape[0], 2, 0.5, s)
+    torch.cuda.synchronize()
+    seeded = [bt.dump_msgs(k) for k in range(len(insts))]
+    _, ll = run_batch(torch, bt, datas, inits)
+    for k, x in enumerate(insts):
+        m = seeded[k].copy()
+        H, W = x["H"], x["W"]
+        assert not m[:, :, 0, :].any() and not m[:, :, :, 0].any() and not m[:, :, H - 1, :].any()
+        bel = np.zeros_like(datas[k])
+        for it in range(iters):
+            oracle_mod.half_sweep(datas[k], m, bel, x["table"], x["model"], 0, it)
+        want = oracle_mod.labels_from_belief(bel, np.zeros((H, W), np.int32))
+        np.testing.assert_array_equal(ll[k].cpu().numpy()[1:-1, 1:-1], want[1:-1, 1:-1], err_msg=f"inst {k}")
+        np.testing.assert_array_equal(bt.dump_msgs(k), m, err_msg=f"inst {k}")
