@@ -222,22 +222,28 @@ __global__ void k_flip_weights(const uint16_t *__restrict__ w, int Cout, int Cin
 }
 
 // Warm-start input: costs [L][H][W] (reference layout) min-pooled over f x f
-// blocks, scaled, written NHWC bf16 [h][w][L] for instance slot `n`.
+// blocks, scaled, written NHWC bf16 [h][w][L] for instance slot `n`.  One
+// thread per pooled cell with X fastest, so a wave reads 64 adjacent f-wide
+// row segments of one label plane (coalesced); the bf16 writes are strided by
+// L but the output is f*f*2 times smaller than the input.
 template <typename T>
 __global__ void k_minpool(const T *__restrict__ d, int W, int H, int L, int f, float scale, int w4, int h4,
                           uint16_t *__restrict__ out) {
     const int64_t n = (int64_t)w4 * h4 * L;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int l = (int)(i % L);
-        const int X = (int)((i / L) % w4);
-        const int Y = (int)(i / ((int64_t)L * w4));
+        const int X = (int)(i % w4);
+        const int64_t r = i / w4;
+        const int Y = (int)(r % h4);
+        const int l = (int)(r / h4);
+        const int x1 = min(X * f + f, W), y1 = min(Y * f + f, H);
+        const T *plane = d + (int64_t)l * H * W;
         float mn = 3.4e38f;
-        for (int yy = Y * f; yy < Y * f + f && yy < H; ++yy)
-            for (int xx = X * f; xx < X * f + f && xx < W; ++xx) {
-                const float v = (float)d[((int64_t)l * H + yy) * W + xx];
+        for (int yy = Y * f; yy < y1; ++yy)
+            for (int xx = X * f; xx < x1; ++xx) {
+                const float v = (float)plane[(int64_t)yy * W + xx];
                 mn = v < mn ? v : mn;
             }
-        out[i] = f2bf(mn * scale);
+        out[((int64_t)Y * w4 + X) * L + l] = f2bf(mn * scale);
     }
 }
 
@@ -272,23 +278,42 @@ struct InjectArgs {
     float inv_scale;
 };
 
+// One thread per (register plane, slot) of the outbox array of colour
+// blockIdx.y & 1 and direction blockIdx.y >> 1, writing that slot's G labels
+// contiguously (a 16-byte store for the quad layout), so the writes of a wave
+// are one contiguous run.  Border nodes get 0; padding slots and padding
+// labels are left as they are.
 __global__ void k_inject(const float *__restrict__ pred, InjectArgs a) {
-    const int64_t n = (int64_t)a.g.w * a.g.h * 4 * a.L;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int l = (int)(i % a.L);
-        const int d = (int)((i / a.L) % 4);
-        const int64_t node = i / (4 * a.L);
-        const int x = (int)(node % a.g.w), y = (int)(node / a.g.w);
-        if (x < 1 || y < 1 || x > a.g.w - 2 || y > a.g.h - 2) continue;
-        int X = x >> a.shift, Y = y >> a.shift;
-        X = X < a.pw ? X : a.pw - 1;
-        Y = Y < a.ph ? Y : a.ph - 1;
-        const float v = pred[((int64_t)Y * a.pw + X) * 4 * a.L + d * a.L + l] * a.inv_scale;
-        const int c = (x + y) & 1;
-        const int64_t slot = (int64_t)y * a.g.whp + (x >> 1);
-        const int q = a.interleave ? l % a.G : l / a.C;
-        const int ii = a.interleave ? l / a.G : l - q * a.C;
-        ((float *)a.msgs[c][d])[((int64_t)ii * a.g.nplane + slot) * a.G + q] = v;
+    const int c = blockIdx.y & 1, d = blockIdx.y >> 1;
+    float *__restrict__ out = (float *)a.msgs[c][d];
+    const int64_t n = (int64_t)a.C * a.g.nplane;
+    const int G = a.G, C = a.C, L = a.L;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int ii = (int)(e / a.g.nplane);
+        const int slot = (int)(e - (int64_t)ii * a.g.nplane);
+        const int y = slot / a.g.whp;
+        if (y >= a.g.h) continue;
+        const int x = 2 * (slot - y * a.g.whp) + ((c ^ y) & 1);
+        if (x >= a.g.w) continue;
+        const bool interior = x >= 1 && y >= 1 && x <= a.g.w - 2 && y <= a.g.h - 2;
+        const int X = min(x >> a.shift, a.pw - 1), Y = min(y >> a.shift, a.ph - 1);
+        const float *p = pred + ((int64_t)Y * a.pw + X) * 4 * L + d * L;
+        float *o = out + e * G;
+        if (a.interleave && G == 4 && ii * 4 + 3 < L) {
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (interior) {
+                v.x = p[ii * 4 + 0] * a.inv_scale;
+                v.y = p[ii * 4 + 1] * a.inv_scale;
+                v.z = p[ii * 4 + 2] * a.inv_scale;
+                v.w = p[ii * 4 + 3] * a.inv_scale;
+            }
+            *(float4 *)o = v;
+        } else {
+            for (int q = 0; q < G; ++q) {
+                const int l = a.interleave ? ii * G + q : q * C + ii;
+                if (l < L) o[q] = interior ? p[l] * a.inv_scale : 0.f;
+            }
+        }
     }
 }
 
@@ -451,8 +476,7 @@ hipError_t launch_inject(const float *pred, void *const msgs[2][4], const LevelG
     a.pw = pw;
     a.ph = ph;
     a.inv_scale = inv_scale;
-    const int64_t n = (int64_t)g.w * g.h * 4 * L;
-    hipLaunchKernelGGL(k_inject, dim3(nblk(n)), dim3(256), 0, st, pred, a);
+    hipLaunchKernelGGL(k_inject, dim3(nblk((int64_t)C * g.nplane), 8), dim3(256), 0, st, pred, a);
     return hipGetLastError();
 }
 

@@ -209,3 +209,71 @@ def test_training_lowers_loss():
     first, last = hist[0][1], hist[-1][1]
     say(f"  loss {first:.4f} -> {last:.4f}")
     assert last < 0.7 * first
+
+
+def test_trained_net_beats_zero_seed():
+    """Ablation with the committed 640x480x64 truncated-linear network: seeding
+    the finest level from its prediction and running 10 half-sweeps gives a
+    lower energy than the same 10 half-sweeps from zero messages."""
+    torch = torch_cuda()
+    from gridinfcuda_amd.warmstart import ConvNet, Pipeline, WarmStart, Workload, default_model_path
+    say("trained network vs zero seed")
+    wl = Workload()
+    net = ConvNet.load(default_model_path(wl))
+    pipe = Pipeline(wl)
+    ws_net, ws_zero = WarmStart(wl, net), WarmStart(wl, None)
+    lab = torch.zeros((wl.H, wl.W), dtype=torch.int32, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    for seed in (2_000_000, 2_000_001):
+        g = G.gen_stereo(seed, wl.W, wl.H, wl.L, wl.rects, wl.L - 1, wl.trunc, want=("left", "right"))
+        pipe.load_costs(g["left"], g["right"])
+        pipe.plain()
+        e_plain = pipe.energy()
+        es = []
+        for ws in (ws_net, ws_zero):
+            lab.zero_()
+            ws.run(pipe.costs, lab, [10], seed_level=0)
+            es.append(G.energy_f64(wl.W, wl.H, wl.L, G.F32, pipe.costs.data_ptr(), pipe.table_dev.data_ptr(),
+                                   lab.data_ptr(), s))
+        say(f"  seed {seed}: excess over plain HBP: net {es[0] / e_plain - 1:.4f}, zero seed {es[1] / e_plain - 1:.4f}")
+        assert es[0] < es[1]
+
+
+def test_batched_warm_start_pipeline():
+    """Batched inference + seeded batched BP: every instance's labels equal a
+    single-plan seeded run from the same prediction, and the fallback takes
+    the plain HBP labels exactly when the energy rule says so."""
+    torch = torch_cuda()
+    from gridinfcuda_amd.warmstart import ConvNet, Workload
+    from gridinfcuda_amd.warmstart_batch import BatchedWarmStart, fallback_flags
+    say("batched warm start pipeline")
+    classes = [Workload(W=96, H=64, L=32, model=G.SM_TRUNC_LINEAR, lam=4, tau=3, rects=6, levels=3, iters=5),
+               Workload(W=70, H=50, L=64, model=G.SM_POTTS, lam=15, tau=1, rects=4, levels=3, iters=5)]
+    nets = [ConvNet(32, seed=1), ConvNet(64, seed=2)]
+    cc = [0, 1, 1, 0, 1]
+    K = 6
+    bws = BatchedWarmStart(classes, nets, cc, K)
+    bws.load([70 + k for k in range(len(cc))])
+    bws.plain()
+    bws.warm(True)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream().cuda_stream
+    for c, idx in bws.members.items():
+        wl = classes[c]
+        plan = G.Plan(G.make_desc(wl.W, wl.H, wl.L, G.F32, wl.model, 1, K), wl.table)
+        for j, k in enumerate(idx):
+            lab = torch.zeros((wl.H, wl.W), dtype=torch.int32, device="cuda")
+            plan.seed(0, bws.pred[c][j].data_ptr(), wl.w4, wl.h4, 2, wl.msg_scale, s)
+            plan.run_schedule(bws.costs[k].data_ptr(), lab.data_ptr(), 0, [K], True, s)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(bws.labels[k].cpu().numpy(), lab.cpu().numpy(), err_msg=f"instance {k}")
+    e_plain = bws.energies(bws.plain_labels)
+    e_warm = bws.energies(bws.labels)
+    warm = [t.cpu().numpy().copy() for t in bws.labels]
+    flags = bws.finalize(e_warm, e_plain)
+    assert flags == fallback_flags(e_warm, e_plain)
+    for k in range(len(cc)):
+        want = bws.plain_labels[k].cpu().numpy() if flags[k] else warm[k]
+        np.testing.assert_array_equal(bws.labels[k].cpu().numpy(), want)
+    assert fallback_flags([1.0011, 1.0009], [1.0, 1.0]) == [True, False]
+    bws.close()
