@@ -124,6 +124,59 @@ def bench_batched(args) -> dict:
     }
 
 
+def bench_warmstart(args) -> dict:
+    """Learned warm start, 640x480, 64 labels, TL lambda=10 tau=3: per seeded
+    evaluation instance (seeds 2000000...), plain HBP 5x10 (the judge) against
+    each seeded schedule, with the committed network and with zero seeds.
+    value = time of the cheapest schedule that reaches plain HBP's energy
+    within 0.1 % on every instance, divided by plain HBP's time."""
+    import torch
+    from gridinfcuda_amd import warmstart as WS
+    wl = WS.Workload()
+    net = WS.ConvNet.load(WS.default_model_path(wl))
+    seeds = list(range(2_000_000, 2_000_000 + args.eval_instances))
+    log = lambda m: print(m, file=sys.stderr)
+    rows = {"net": WS.summarize(WS.evaluate(wl, net, seeds, reps=args.steps, log=log)),
+            "zero_seed": WS.summarize(WS.evaluate(wl, None, seeds, reps=args.steps, log=log))}
+
+    def best(rs):
+        ok = [r for r in rs if r["within_0.1pct"] >= 1.0]
+        return min(ok, key=lambda r: r["time_ratio"]) if ok else None
+    b_net, b_zero = best(rows["net"]), best(rows["zero_seed"])
+    return {
+        "metric": "ms to plain-BP energy, warm-start vs plain", "value": b_net["time_ratio"] if b_net else None,
+        "unit": "x plain HBP time", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": None,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 BP, bf16 network",
+        "data": f"synthetic (seeded stereo generator, seeds {seeds[0]}..{seeds[-1]})",
+        "config": {"workload": "learned warm start 640x480x64 TL lambda=10 tau=3 vs plain HBP 5 levels x 10",
+                   "instances": len(seeds), "target": "energy within 0.1% of plain HBP on every instance"},
+        "best_with_net": b_net, "best_zero_seed": b_zero, "schedules": rows,
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def bench_warmstart_batch(args) -> dict:
+    """Batched learned warm start: 256 instances (640x480x64 / 1280x720x96,
+    TL and Potts), batched network inference, seeded batched finest-level BP
+    of K half-sweeps, fallback to plain HBP above 0.1 % energy excess."""
+    from gridinfcuda_amd import warmstart_batch as WB
+    ks = [int(v) for v in args.batch_iters.split(",")]
+    res = WB.evaluate_batched(args.eval_instances, iters=ks, reps=max(1, args.steps),
+                              log=lambda m: print(m, file=sys.stderr))
+    s = min(res["summary"], key=lambda r: r["time_ratio"])
+    return {
+        "metric": "ms to plain-BP energy, warm-start vs plain", "value": s["time_ratio"], "unit": "x plain HBP time",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": s["warm_total_ms"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 BP, bf16 network",
+        "data": f"synthetic (seeded stereo generator, seeds 5000..{5000 + args.eval_instances - 1})",
+        "config": {"workload": "batched learned warm start, 640x480x64 and 1280x720x96, TL and Potts, "
+                               "fallback to plain HBP 5x10 above 0.1% energy excess",
+                   "instances": args.eval_instances, "chunk": res["chunk"], "tiles": res["tiles"]},
+        "fallback_fraction": s["fallback_fraction"], "summary": res["summary"],
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,9 +191,15 @@ def main() -> int:
     ap.add_argument("--norm", type=int, default=0, help="0 mean (reference), 1 min")
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 chunked, 2 quad (per-direction lanes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-api", action="store_true", help="skip the host-entry (PCIe-inclusive) timing")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
-    ap.add_argument("--workload", default="hbp", choices=["hbp", "batched"],
-                    help="hbp: 1920x1080x128 hierarchical BP (default); batched: 64 mixed MRFs")
+    ap.add_argument("--workload", default="hbp", choices=["hbp", "batched", "warmstart", "warmstart-batch"],
+                    help="hbp: 1920x1080x128 hierarchical BP (default); batched: 64 mixed MRFs; "
+                         "warmstart: 640x480x64 learned warm start vs plain HBP; "
+                         "warmstart-batch: 256-instance batched warm start with fallback")
+    ap.add_argument("--eval-instances", type=int, default=None,
+                    help="warm-start workloads: instances evaluated (default 8 / 256)")
+    ap.add_argument("--batch-iters", default="16", help="warmstart-batch: finest-level half-sweeps K (list)")
     ap.add_argument("--retune", action="store_true", help="batched: rerun the tile search and rewrite the cache")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (profiler counter calibration)")
@@ -160,6 +219,14 @@ def main() -> int:
         return 1
     if args.workload == "batched":
         print(json.dumps(bench_batched(args)))
+        return 0
+    if args.workload == "warmstart":
+        args.eval_instances = args.eval_instances or 8
+        print(json.dumps(bench_warmstart(args)))
+        return 0
+    if args.workload == "warmstart-batch":
+        args.eval_instances = args.eval_instances or 256
+        print(json.dumps(bench_warmstart_batch(args)))
         return 0
     W, H, L = args.width, args.height, args.labels
     gen = G.gen_stereo(2, W, H, L, 64, 127, 40)["cost_f32"]
@@ -253,6 +320,22 @@ def main() -> int:
         },
         "energy_f64": energy,
     }
+    if not args.no_api:
+        # End to end through the synchronous host entry (ginf_hbp_host, what the
+        # C++ templates call): pageable host costs in, labels out, PCIe both
+        # ways.  First call on this problem and a repeat call; never `value`.
+        host = G.gen_stereo(2, W, H, L, 64, 127, 40)["cost_f32"]
+        api_ms = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            hl = G.hbp_host(host, table, G.SM_TRUNC_LINEAR, args.levels, args.iters, args.norm,
+                            lanes_per_node=args.lanes)
+            api_ms.append((time.perf_counter() - t0) * 1000.0)
+        out["api_end_to_end_ms"] = {"first_call": round(api_ms[0], 2), "repeat_call": round(api_ms[1], 2),
+                                    "h2d_bytes": int(host.nbytes),
+                                    "labels_equal_device_run": bool(np.array_equal(hl, labels.cpu().numpy()))}
+        G.lib().ginf_hbp_host_release()
+        del host
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(threads)

@@ -29,7 +29,7 @@ EXPORTS = [
     "ginf_batch_dump_msgs", "ginf_batch_destroy", "ginf_bp_plan_run_schedule", "ginf_bp_plan_seed",
     "ginf_bp_plan_block_mean_msgs", "ginf_conv3x3_fwd", "ginf_conv3x3_bwd_weight", "ginf_conv3x3_flip_weights",
     "ginf_relu_bwd", "ginf_mse", "ginf_adam", "ginf_to_bf16", "ginf_minpool_costs", "ginf_block_mean_msgs",
-    "ginf_stereo_costs", "ginf_batch_set_seeded", "ginf_batch_seed",
+    "ginf_stereo_costs", "ginf_batch_set_seeded", "ginf_batch_seed", "ginf_hbp_host_release",
 ]
 
 

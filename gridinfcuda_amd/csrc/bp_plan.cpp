@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -111,6 +112,36 @@ struct ginf_bp_plan_s {
 namespace {
 
 size_t elem_size(int dtype) { (void)dtype; return 4; }
+
+// ginf_hbp_host keeps the plan and device buffers of its last problem (one
+// entry per process: descriptor, smoothness table and device), so repeated
+// synchronous calls on one shape neither reallocate nor re-zero the level
+// state; a call with anything different replaces the entry.  The mutex
+// serialises callers.  The entry is never freed by a static destructor (the
+// HIP runtime may already be gone at exit); ginf_hbp_host_release frees it.
+struct HostCache {
+    std::mutex mu;
+    ginf_bp_plan plan = nullptr;
+    void *dd = nullptr, *dl = nullptr;
+    ginf_bp_desc desc{};
+    int device = -1;
+    std::vector<unsigned char> table;
+};
+
+HostCache &host_cache() {
+    static HostCache *c = new HostCache;
+    return *c;
+}
+
+void host_cache_release(HostCache &hc) {
+    if (hc.plan) ginf_bp_plan_destroy(hc.plan);
+    if (hc.dd) (void)hipFree(hc.dd);
+    if (hc.dl) (void)hipFree(hc.dl);
+    hc.plan = nullptr;
+    hc.dd = hc.dl = nullptr;
+    hc.device = -1;
+    hc.table.clear();
+}
 
 int validate_desc(const ginf_bp_desc *d) {
     if (!d) return set_error(GINF_ERR_INVALID, "desc is NULL");
@@ -538,26 +569,48 @@ int ginf_bp_plan_dump_data(ginf_bp_plan p, int level, void *data_host) {
 int ginf_hbp_host(const ginf_bp_desc *desc, const void *data_host, const void *sm_table_host, int32_t *labels_host) {
     int rc = validate_desc(desc);
     if (rc) return rc;
-    if (!data_host || !labels_host) return set_error(GINF_ERR_INVALID, "NULL argument");
-    ginf_bp_plan p = nullptr;
-    rc = ginf_bp_plan_create(desc, sm_table_host, &p);
-    if (rc) return rc;
+    if (!data_host || !labels_host || !sm_table_host) return set_error(GINF_ERR_INVALID, "NULL argument");
+    int dev = 0;
+    GINF_HIP_TRY(hipGetDevice(&dev));
     const size_t nd = (size_t)desc->width * desc->height * desc->num_labels * 4;
     const size_t nl = (size_t)desc->width * desc->height * 4;
-    void *dd = nullptr, *dl = nullptr;
-    hipError_t e = hipMalloc(&dd, nd);
-    if (e == hipSuccess) e = hipMalloc(&dl, nl);
-    if (e == hipSuccess) e = hipMemcpy(dd, data_host, nd, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dl, labels_host, nl, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        rc = plan_run(p, dd, (int32_t *)dl, nullptr);
-        if (rc == GINF_OK) e = hipMemcpy(labels_host, dl, nl, hipMemcpyDeviceToHost);
+    const size_t nt = (size_t)desc->num_labels * desc->num_labels * elem_size(desc->dtype);
+    HostCache &hc = host_cache();
+    std::lock_guard<std::mutex> lock(hc.mu);
+    const bool hit = hc.plan && hc.device == dev && std::memcmp(&hc.desc, desc, sizeof(ginf_bp_desc)) == 0 &&
+                     hc.table.size() == nt && std::memcmp(hc.table.data(), sm_table_host, nt) == 0;
+    if (!hit) {
+        host_cache_release(hc);
+        rc = ginf_bp_plan_create(desc, sm_table_host, &hc.plan);
+        if (rc) return rc;
+        hipError_t e = hipMalloc(&hc.dd, nd);
+        if (e == hipSuccess) e = hipMalloc(&hc.dl, nl);
+        if (e != hipSuccess) {
+            host_cache_release(hc);
+            return set_error(GINF_ERR_HIP, "ginf_hbp_host: %s", hipGetErrorString(e));
+        }
+        hc.desc = *desc;
+        hc.device = dev;
+        hc.table.assign((const unsigned char *)sm_table_host, (const unsigned char *)sm_table_host + nt);
     }
-    if (dd) hipFree(dd);
-    if (dl) hipFree(dl);
-    ginf_bp_plan_destroy(p);
-    if (e != hipSuccess) return set_error(GINF_ERR_HIP, "ginf_hbp_host: %s", hipGetErrorString(e));
+    hipError_t e = hipMemcpy(hc.dd, data_host, nd, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(hc.dl, labels_host, nl, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rc = plan_run(hc.plan, hc.dd, (int32_t *)hc.dl, nullptr);
+        if (rc == GINF_OK) e = hipMemcpy(labels_host, hc.dl, nl, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess || rc != GINF_OK) {
+        host_cache_release(hc);
+        if (e != hipSuccess) return set_error(GINF_ERR_HIP, "ginf_hbp_host: %s", hipGetErrorString(e));
+    }
     return rc;
+}
+
+int ginf_hbp_host_release(void) {
+    HostCache &hc = host_cache();
+    std::lock_guard<std::mutex> lock(hc.mu);
+    host_cache_release(hc);
+    return GINF_OK;
 }
 
 int ginf_energy_f64(int width, int height, int num_labels, int dtype, const void *data_dev, const void *table_dev,

@@ -279,18 +279,16 @@ struct InjectArgs {
 };
 
 // One thread per (register plane, slot) of the outbox array of colour
-// blockIdx.y & 1 and direction blockIdx.y >> 1, writing that slot's G labels
-// contiguously (a 16-byte store for the quad layout), so the writes of a wave
-// are one contiguous run.  Border nodes get 0; padding slots and padding
-// labels are left as they are.
+// blockIdx.y & 1 and direction blockIdx.y >> 1 (plane blockIdx.z), writing
+// that slot's G labels contiguously (a 16-byte store for the quad layout), so
+// the writes of a wave are one contiguous run.  Border nodes get 0; padding
+// slots and padding labels are left as they are.
 __global__ void k_inject(const float *__restrict__ pred, InjectArgs a) {
     const int c = blockIdx.y & 1, d = blockIdx.y >> 1;
-    float *__restrict__ out = (float *)a.msgs[c][d];
-    const int64_t n = (int64_t)a.C * a.g.nplane;
+    const int ii = blockIdx.z;
+    float *__restrict__ out = (float *)a.msgs[c][d] + (int64_t)ii * a.g.nplane * a.G;
     const int G = a.G, C = a.C, L = a.L;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        const int ii = (int)(e / a.g.nplane);
-        const int slot = (int)(e - (int64_t)ii * a.g.nplane);
+    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < a.g.nslots; slot += gridDim.x * blockDim.x) {
         const int y = slot / a.g.whp;
         if (y >= a.g.h) continue;
         const int x = 2 * (slot - y * a.g.whp) + ((c ^ y) & 1);
@@ -298,7 +296,7 @@ __global__ void k_inject(const float *__restrict__ pred, InjectArgs a) {
         const bool interior = x >= 1 && y >= 1 && x <= a.g.w - 2 && y <= a.g.h - 2;
         const int X = min(x >> a.shift, a.pw - 1), Y = min(y >> a.shift, a.ph - 1);
         const float *p = pred + ((int64_t)Y * a.pw + X) * 4 * L + d * L;
-        float *o = out + e * G;
+        float *o = out + (int64_t)slot * G;
         if (a.interleave && G == 4 && ii * 4 + 3 < L) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (interior) {
@@ -476,7 +474,8 @@ hipError_t launch_inject(const float *pred, void *const msgs[2][4], const LevelG
     a.pw = pw;
     a.ph = ph;
     a.inv_scale = inv_scale;
-    hipLaunchKernelGGL(k_inject, dim3(nblk((int64_t)C * g.nplane), 8), dim3(256), 0, st, pred, a);
+    if (C > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_inject, dim3(nblk(g.nslots), 8, C), dim3(256), 0, st, pred, a);
     return hipGetLastError();
 }
 

@@ -85,9 +85,15 @@ int ginf_bp_plan_workspace_bytes(ginf_bp_plan plan, size_t *bytes);
 
 /* Synchronous host-pointer entry: the drop-in for decodeHbp/gpuDecodeHbp
  * (decodeBp/gpuDecodeBp with num_levels = 1).  Copies costs to the device,
- * runs, copies interior labels back into labels_host. */
+ * runs, copies interior labels back into labels_host.  The reference
+ * allocates and frees its state on every call (src/bp.cc:89-179,
+ * src/gpu/bp.cu:141-222); here the device state of the last call's problem
+ * (descriptor, table, device) is kept and reused by the next call with the
+ * same problem, and calls are serialised. */
 int ginf_hbp_host(const ginf_bp_desc *desc, const void *data_host, const void *sm_table_host,
                   int32_t *labels_host);
+/* Frees the device state ginf_hbp_host keeps (no reference counterpart). */
+int ginf_hbp_host_release(void);
 
 /* Labeling energy of Grid::getLabelingCost (src/grid.cc:102-125): interior
  * nodes' D(f) + V(f, f_north) + V(f, f_east), summed in FP64 on the device in

@@ -190,3 +190,27 @@ def test_labelling_sweeps_repeatable(oracle_mod, iters):
         lab, msgs, _ = run_plan(data, table, 2, 1, iters, norm=0, dump_level=0)
         assert int((msgs != ref_msgs).sum()) == 0, f"rep {rep}"
         np.testing.assert_array_equal(lab, ref_lab)
+
+
+def test_host_entry_reuses_state_correctly(oracle_mod):
+    """ginf_hbp_host keeps the last problem's device state: repeated calls,
+    a call with another table, another shape and border labels set by the
+    caller all give the oracle's labels."""
+    say("host entry state reuse")
+    L, H, W = 24, 37, 53
+    a = stereo(41, W, H, L, 6, L - 1, 30, np.float32)
+    b = stereo(42, W, H, L, 6, L - 1, 30, np.float32)
+    t1 = oracle_mod.table_trunc_linear(L, 4, 12)
+    t2 = oracle_mod.table_potts(L, 9)
+    rng = np.random.default_rng(3)
+    init = rng.integers(0, L, size=(H, W)).astype(np.int32)
+    cases = [(a, t1, 2), (b, t1, 2), (a, t2, 1), (a, t1, 2)]
+    for k, (data, table, model) in enumerate(cases):
+        want = oracle_mod.hbp(data, table, model, 3, 4, norm=0, labels_init=init)
+        got = G.hbp_host(data, table, model, 3, 4, norm=0, labels_init=init)
+        np.testing.assert_array_equal(got, want, err_msg=f"call {k}")
+    c = stereo(43, 31, 20, 16, 4, 15, 20, np.float32)
+    tc = oracle_mod.table_trunc_linear(16, 4, 12)
+    np.testing.assert_array_equal(G.hbp_host(c, tc, 2, 2, 5), oracle_mod.hbp(c, tc, 2, 2, 5))
+    assert G.lib().ginf_hbp_host_release() == 0
+    np.testing.assert_array_equal(G.hbp_host(c, tc, 2, 2, 5), oracle_mod.hbp(c, tc, 2, 2, 5))
