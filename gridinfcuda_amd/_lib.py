@@ -30,6 +30,7 @@ EXPORTS = [
     "ginf_bp_plan_block_mean_msgs", "ginf_conv3x3_fwd", "ginf_conv3x3_bwd_weight", "ginf_conv3x3_flip_weights",
     "ginf_relu_bwd", "ginf_mse", "ginf_adam", "ginf_to_bf16", "ginf_minpool_costs", "ginf_block_mean_msgs",
     "ginf_stereo_costs", "ginf_batch_set_seeded", "ginf_batch_seed", "ginf_hbp_host_release",
+    "ginf_icm", "ginf_max_prior", "ginf_icm_host", "ginf_max_prior_host",
 ]
 
 
@@ -222,6 +223,40 @@ def hbp_host(data: np.ndarray, table: np.ndarray, sm_model: int, levels: int, it
     check(lib().ginf_hbp_host(ctypes.byref(d), data.ctypes.data_as(ctypes.c_void_p),
                               table.ctypes.data_as(ctypes.c_void_p), labels.ctypes.data_as(ctypes.c_void_p)))
     return labels
+
+
+ICM_SYNC, ICM_ASYNC = 0, 1
+
+
+def icm(width, height, num_labels, dtype, data_ptr, table_ptr, initial_ptr, iters, mode, result_ptr,
+        stream=0) -> None:
+    """ICM on device pointers (ginf_icm)."""
+    vp = ctypes.c_void_p
+    check(lib().ginf_icm(width, height, num_labels, dtype, vp(data_ptr), vp(table_ptr), vp(initial_ptr), iters, mode,
+                         vp(result_ptr), vp(stream)))
+
+
+def icm_host(data: np.ndarray, table: np.ndarray, initial: np.ndarray, iters: int, mode: int = ICM_SYNC) -> np.ndarray:
+    """Synchronous host-array ICM (ginf_icm_host); data is (L, H, W)."""
+    data = np.ascontiguousarray(data)
+    L, H, W = data.shape
+    table = np.ascontiguousarray(table, dtype=data.dtype)
+    init = np.ascontiguousarray(initial, dtype=np.int32)
+    out = np.zeros((H, W), np.int32)
+    vp = ctypes.c_void_p
+    check(lib().ginf_icm_host(W, H, L, dtype_code(data.dtype), vp(data.ctypes.data), vp(table.ctypes.data),
+                              vp(init.ctypes.data), iters, mode, vp(out.ctypes.data)))
+    return out
+
+
+def max_prior_host(data: np.ndarray) -> np.ndarray:
+    """decodeMaxPrior on the device (ginf_max_prior_host); data is (L, H, W)."""
+    data = np.ascontiguousarray(data)
+    L, H, W = data.shape
+    out = np.zeros((H, W), np.int32)
+    check(lib().ginf_max_prior_host(W, H, L, dtype_code(data.dtype), ctypes.c_void_p(data.ctypes.data),
+                                    ctypes.c_void_p(out.ctypes.data)))
+    return out
 
 
 def energy_f64(width, height, num_labels, dtype, data_ptr, table_ptr, labels_ptr, stream=0) -> float:

@@ -203,4 +203,55 @@ template void decodeHbp<int>(Grid<int> *, int, int, Matrix<int> *);
 template void gpuDecodeHbp<float>(Grid<float> *, int, int, Matrix<int> *);
 template void gpuDecodeHbp<int>(Grid<int> *, int, int, Matrix<int> *);
 
+// ------------------------------------------------------------------ ICM, estimators
+
+namespace {
+
+template <typename T>
+void run_icm(const char *who, Grid<T> *grid, Matrix<int> *initial, int iters, int mode, Matrix<int> *result) {
+    const int rc = ginf_icm_host(grid->getWidth(), grid->getHeight(), grid->getNumLabels(), dtype_of<T>(),
+                                 grid->getDataCosts()->data, grid->getSmCosts()->data, initial->data, iters, mode,
+                                 result->data);
+    if (rc != GINF_OK) {
+        fprintf(stderr, "%s failed (code %d): %s\n", who, rc, ginf_last_error());
+        exit(1);
+    }
+}
+
+}  // namespace
+
+template <typename T>
+void decodeIcmSync(Grid<T> *grid, Matrix<int> *initial, int numIters, Matrix<int> *result) {
+    run_icm("ginf::decodeIcmSync", grid, initial, numIters, GINF_ICM_SYNC, result);
+}
+
+template <typename T>
+void gpuDecodeIcm(Grid<T> *grid, Matrix<int> *initial, int numIters, Matrix<int> *result) {
+    run_icm("ginf::gpuDecodeIcm", grid, initial, numIters, GINF_ICM_SYNC, result);
+}
+
+template <typename T>
+void decodeIcmAsync(Grid<T> *grid, Matrix<int> *initial, int numIters, Matrix<int> *result) {
+    run_icm("ginf::decodeIcmAsync", grid, initial, numIters, GINF_ICM_ASYNC, result);
+}
+
+template <typename T>
+void decodeMaxPrior(Grid<T> *grid, Matrix<int> *result) {
+    const int rc = ginf_max_prior_host(grid->getWidth(), grid->getHeight(), grid->getNumLabels(), dtype_of<T>(),
+                                       grid->getDataCosts()->data, result->data);
+    if (rc != GINF_OK) {
+        fprintf(stderr, "ginf::decodeMaxPrior failed (code %d): %s\n", rc, ginf_last_error());
+        exit(1);
+    }
+}
+
+template void decodeIcmSync<float>(Grid<float> *, Matrix<int> *, int, Matrix<int> *);
+template void decodeIcmSync<int>(Grid<int> *, Matrix<int> *, int, Matrix<int> *);
+template void gpuDecodeIcm<float>(Grid<float> *, Matrix<int> *, int, Matrix<int> *);
+template void gpuDecodeIcm<int>(Grid<int> *, Matrix<int> *, int, Matrix<int> *);
+template void decodeIcmAsync<float>(Grid<float> *, Matrix<int> *, int, Matrix<int> *);
+template void decodeIcmAsync<int>(Grid<int> *, Matrix<int> *, int, Matrix<int> *);
+template void decodeMaxPrior<float>(Grid<float> *, Matrix<int> *);
+template void decodeMaxPrior<int>(Grid<int> *, Matrix<int> *);
+
 }  // namespace ginf

@@ -211,6 +211,27 @@ int ginf_batch_seed(ginf_batch batch, int instance, const float *pred_dev, int p
 int ginf_batch_dump_msgs(ginf_batch batch, int instance, void *msgs_host);
 int ginf_batch_destroy(ginf_batch batch);
 
+/* ---- ICM and the max-prior labelling (decoders adjacent to BP) ----------
+ * ICM: decodeIcmSync / gpuDecodeIcm (synchronous, src/icm.cc:5-37,
+ * src/gpu/icm.cu:21-83) and decodeIcmAsync (in place, row-major,
+ * src/icm.cc:39-62).  Interior nodes move to the lowest-index label of
+ * minimal D(l) + V(l,f_S) + V(l,f_E) + V(l,f_N) + V(l,f_W) (T arithmetic,
+ * that order); border labels stay as in `initial`.  Device pointers: costs
+ * [L][H][W], table V(a,b) at a + L*b, labels [H][W]; result may equal
+ * initial.  Stream-ordered. */
+enum ginf_icm_mode { GINF_ICM_SYNC = 0, GINF_ICM_ASYNC = 1 };
+int ginf_icm(int width, int height, int num_labels, int dtype, const void *data_dev, const void *table_dev,
+             const int32_t *initial_dev, int iters, int mode, int32_t *result_dev, void *stream);
+/* decodeMaxPrior (src/include/estimators.h:10-25): every node's lowest-index
+ * label of minimal data cost. */
+int ginf_max_prior(int width, int height, int num_labels, int dtype, const void *data_dev, int32_t *result_dev,
+                   void *stream);
+/* Synchronous host-pointer forms (used by include/icm.h and estimators.h). */
+int ginf_icm_host(int width, int height, int num_labels, int dtype, const void *data_host, const void *table_host,
+                  const int32_t *initial_host, int iters, int mode, int32_t *result_host);
+int ginf_max_prior_host(int width, int height, int num_labels, int dtype, const void *data_host,
+                        int32_t *result_host);
+
 const char *ginf_last_error(void);
 
 /* Library version string. */

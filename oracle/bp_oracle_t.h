@@ -229,6 +229,64 @@ double OFN(oracle_energy_f64)(const OT *data, const OT *table, const int *labels
     return e;
 }
 
+/* Grid::getLabelingCost(f, x, y, label) (src/grid.cc:127-136): D(label) and
+ * then V(label, f(p + dir d)) for d = S, E, N, W, accumulated in T. */
+static OT OFN(local_cost)(const OT *data, const OT *table, const int *f, int W, int H, int L, int x, int y,
+                          int label) {
+    OT c = data[OFN(d_idx)(W, H, x, y, label)];
+    for (int d = 0; d < 4; d++) c += table[label + L * f[(size_t)(y + DIR_Y[d]) * W + x + DIR_X[d]]];
+    return c;
+}
+
+/* Lowest-index label of minimal local cost (strict <, src/icm.cc:18-25). */
+static int OFN(icm_best)(const OT *data, const OT *table, const int *f, int W, int H, int L, int x, int y) {
+    OT best = OFN(local_cost)(data, table, f, W, H, L, x, y, 0);
+    int arg = 0;
+    for (int i = 1; i < L; i++) {
+        OT c = OFN(local_cost)(data, table, f, W, H, L, x, y, i);
+        if (c < best) { best = c; arg = i; }
+    }
+    return arg;
+}
+
+/* decodeIcmSync (src/icm.cc:5-37): every interior node moves to its best
+ * label given the previous iteration's labelling; borders keep `initial`. */
+void OFN(oracle_icm_sync)(const OT *data, const OT *table, int W, int H, int L, const int *initial, int iters,
+                          int *result) {
+    size_t n = (size_t)W * H;
+    int *old = (int *)malloc(n * sizeof(int));
+    memcpy(result, initial, n * sizeof(int));
+    memcpy(old, result, n * sizeof(int));
+    for (int t = 0; t < iters; t++) {
+        for (int y = 1; y < H - 1; y++)
+            for (int x = 1; x < W - 1; x++) result[(size_t)y * W + x] = OFN(icm_best)(data, table, old, W, H, L, x, y);
+        memcpy(old, result, n * sizeof(int));
+    }
+    free(old);
+}
+
+/* decodeIcmAsync (src/icm.cc:39-62): the same update in place, row-major. */
+void OFN(oracle_icm_async)(const OT *data, const OT *table, int W, int H, int L, const int *initial, int iters,
+                           int *result) {
+    memcpy(result, initial, (size_t)W * H * sizeof(int));
+    for (int t = 0; t < iters; t++)
+        for (int y = 1; y < H - 1; y++)
+            for (int x = 1; x < W - 1; x++)
+                result[(size_t)y * W + x] = OFN(icm_best)(data, table, result, W, H, L, x, y);
+}
+
+/* decodeMaxPrior (src/include/estimators.h:10-25): every node (borders
+ * included) takes the lowest-index label of minimal data cost. */
+void OFN(oracle_max_prior)(const OT *data, int W, int H, int L, int *result) {
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            int m = 0;
+            for (int i = 1; i < L; i++)
+                if (data[OFN(d_idx)(W, H, x, y, i)] < data[OFN(d_idx)(W, H, x, y, m)]) m = i;
+            result[(size_t)y * W + x] = m;
+        }
+}
+
 #undef OFN
 #undef OCAT
 #undef OCAT2

@@ -60,6 +60,9 @@ def _declare(L: ctypes.CDLL) -> None:
         e64 = getattr(L, f"oracle_energy_f64_{sfx}")
         e64.argtypes = [P, P, P, i, i, i]
         e64.restype = ctypes.c_double
+        for name in ("oracle_icm_sync", "oracle_icm_async"):
+            getattr(L, f"{name}_{sfx}").argtypes = [P, P, i, i, i, P, i, P]
+        getattr(L, f"oracle_max_prior_{sfx}").argtypes = [P, i, i, i, P]
     L.oracle_max_threads.restype = ctypes.c_int
 
 
@@ -154,6 +157,27 @@ def energy_f64(data: np.ndarray, table: np.ndarray, labels: np.ndarray) -> float
     labels = np.ascontiguousarray(labels, dtype=np.int32)
     L, H, W = data.shape
     return getattr(lib(), f"oracle_energy_f64_{_sfx(data.dtype)}")(_ptr(data), _ptr(table), _ptr(labels), W, H, L)
+
+
+def icm(data: np.ndarray, table: np.ndarray, initial: np.ndarray, iters: int, sync: bool = True) -> np.ndarray:
+    """decodeIcmSync / decodeIcmAsync (src/icm.cc:5-62)."""
+    data = np.ascontiguousarray(data)
+    table = np.ascontiguousarray(table, dtype=data.dtype)
+    init = np.ascontiguousarray(initial, dtype=np.int32)
+    L, H, W = data.shape
+    out = np.zeros((H, W), np.int32)
+    fn = getattr(lib(), f"oracle_icm_{'sync' if sync else 'async'}_{_sfx(data.dtype)}")
+    fn(_ptr(data), _ptr(table), W, H, L, _ptr(init), iters, _ptr(out))
+    return out
+
+
+def max_prior(data: np.ndarray) -> np.ndarray:
+    """decodeMaxPrior (src/include/estimators.h:10-25)."""
+    data = np.ascontiguousarray(data)
+    L, H, W = data.shape
+    out = np.zeros((H, W), np.int32)
+    getattr(lib(), f"oracle_max_prior_{_sfx(data.dtype)}")(_ptr(data), W, H, L, _ptr(out))
+    return out
 
 
 def max_threads() -> int:

@@ -1,0 +1,68 @@
+"""GPU parity of ICM (synchronous and asynchronous) and the max-prior
+labelling against the oracle, bit for bit, through the C ABI."""
+import numpy as np
+import pytest
+
+from gpu_util import torch_cuda, say
+from gridinfcuda_amd import _lib as G
+
+pytestmark = pytest.mark.gpu
+
+
+def case(seed, L, H, W, dtype, kind):
+    rng = np.random.default_rng(seed)
+    data = rng.integers(0, 30, size=(L, H, W)).astype(dtype)
+    a = np.arange(L)
+    if kind == "explicit":
+        table = rng.integers(0, 20, size=(L, L)).astype(dtype)
+    elif kind == "potts":
+        table = (9 * (a[:, None] != a[None, :])).astype(dtype)
+    else:
+        table = np.minimum(5 * np.abs(a[:, None] - a[None, :]), 15).astype(dtype)
+    init = rng.integers(0, L, size=(H, W)).astype(np.int32)
+    return data, table, init
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float32])
+@pytest.mark.parametrize("mode", [G.ICM_SYNC, G.ICM_ASYNC])
+@pytest.mark.parametrize("L,H,W,iters,kind", [(16, 37, 53, 5, "tl"), (7, 3, 40, 3, "explicit"), (5, 30, 3, 4, "potts"),
+                                              (32, 129, 300, 2, "explicit"), (3, 2, 9, 3, "tl"), (9, 17, 11, 0, "tl"),
+                                              (64, 90, 70, 6, "potts"), (4, 400, 20, 7, "explicit")])
+def test_icm_matches_oracle(oracle_mod, dtype, mode, L, H, W, iters, kind):
+    torch = torch_cuda()
+    say(f"icm mode={mode} {np.dtype(dtype).name} L={L} H={H} W={W} iters={iters} {kind}")
+    data, table, init = case(L * 7 + H, L, H, W, dtype, kind)
+    want = oracle_mod.icm(data, table, init, iters, sync=(mode == G.ICM_SYNC))
+    dd, td, idev = (torch.from_numpy(a).cuda() for a in (data, table, init))
+    out = torch.full((H, W), -1, dtype=torch.int32, device="cuda")
+    G.icm(W, H, L, G.dtype_code(data.dtype), dd.data_ptr(), td.data_ptr(), idev.data_ptr(), iters, mode,
+          out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), want)
+    # in place (result == initial) and through the host entry
+    G.icm(W, H, L, G.dtype_code(data.dtype), dd.data_ptr(), td.data_ptr(), idev.data_ptr(), iters, mode,
+          idev.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(idev.cpu().numpy(), want)
+    np.testing.assert_array_equal(G.icm_host(data, table, init, iters, mode), want)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float32])
+def test_max_prior_matches_oracle(oracle_mod, dtype):
+    rng = np.random.default_rng(9)
+    data = rng.integers(0, 5, size=(12, 33, 47)).astype(dtype)
+    np.testing.assert_array_equal(G.max_prior_host(data), oracle_mod.max_prior(data))
+
+
+def test_icm_refines_bp_labels(oracle_mod):
+    """BP labels as ICM's initial labelling (the reference's intended chain):
+    asynchronous ICM never raises the energy and equals the oracle."""
+    say("bp -> icm")
+    W, H, L = 96, 64, 32
+    data = G.gen_stereo(17, W, H, L, 8, L - 1, 30)["cost_f32"]
+    a = np.arange(L)
+    table = np.minimum(6 * np.abs(a[:, None] - a[None, :]), 18).astype(np.float32)
+    lab = G.hbp_host(data, table, G.SM_TRUNC_LINEAR, 3, 6)
+    refined = G.icm_host(data, table, lab, 5, G.ICM_ASYNC)
+    np.testing.assert_array_equal(refined, oracle_mod.icm(data, table, lab, 5, sync=False))
+    assert oracle_mod.energy_f64(data, table, refined) <= oracle_mod.energy_f64(data, table, lab)
