@@ -30,7 +30,8 @@ EXPORTS = [
     "ginf_bp_plan_block_mean_msgs", "ginf_conv3x3_fwd", "ginf_conv3x3_bwd_weight", "ginf_conv3x3_flip_weights",
     "ginf_relu_bwd", "ginf_mse", "ginf_adam", "ginf_to_bf16", "ginf_minpool_costs", "ginf_block_mean_msgs",
     "ginf_stereo_costs", "ginf_batch_set_seeded", "ginf_batch_seed", "ginf_hbp_host_release",
-    "ginf_icm", "ginf_max_prior", "ginf_icm_host", "ginf_max_prior_host",
+    "ginf_icm", "ginf_max_prior", "ginf_icm_host", "ginf_max_prior_host", "ginf_alpha_beta",
+    "ginf_alpha_beta_host",
 ]
 
 
@@ -246,6 +247,28 @@ def icm_host(data: np.ndarray, table: np.ndarray, initial: np.ndarray, iters: in
     vp = ctypes.c_void_p
     check(lib().ginf_icm_host(W, H, L, dtype_code(data.dtype), vp(data.ctypes.data), vp(table.ctypes.data),
                               vp(init.ctypes.data), iters, mode, vp(out.ctypes.data)))
+    return out
+
+
+def alpha_beta(width, height, num_labels, data_ptr, table_ptr, initial_ptr, iters, result_ptr, stream=0) -> int:
+    """Alpha-beta swap on device pointers (ginf_alpha_beta, int costs); returns the kept moves."""
+    vp = ctypes.c_void_p
+    moves = ctypes.c_int()
+    check(lib().ginf_alpha_beta(width, height, num_labels, I32, vp(data_ptr), vp(table_ptr), vp(initial_ptr), iters,
+                                vp(result_ptr), ctypes.byref(moves), vp(stream)))
+    return moves.value
+
+
+def alpha_beta_host(data: np.ndarray, table: np.ndarray, initial: np.ndarray, iters: int) -> np.ndarray:
+    """Synchronous host-array alpha-beta swap (ginf_alpha_beta_host); data is (L, H, W) int32."""
+    data = np.ascontiguousarray(data)
+    L, H, W = data.shape
+    table = np.ascontiguousarray(table, dtype=data.dtype)
+    init = np.ascontiguousarray(initial, dtype=np.int32)
+    out = np.zeros((H, W), np.int32)
+    vp = ctypes.c_void_p
+    check(lib().ginf_alpha_beta_host(W, H, L, dtype_code(data.dtype), vp(data.ctypes.data), vp(table.ctypes.data),
+                                     vp(init.ctypes.data), iters, vp(out.ctypes.data)))
     return out
 
 

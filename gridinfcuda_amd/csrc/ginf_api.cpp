@@ -1,4 +1,5 @@
-// C++ drop-in API (include/matrix.h, grid.h, bp.h) over the C ABI.
+// C++ drop-in API (include/matrix.h, grid.h, bp.h, icm.h, estimators.h,
+// graphcut.h) over the C ABI.
 // Matrix and Grid keep the reference host data model (src/matrix.cc,
 // src/grid.cc); the four BP templates run the HIP path via ginf_hbp_host.
 #include <cstdarg>
@@ -7,7 +8,10 @@
 #include <cstring>
 
 #include "bp.h"
+#include "estimators.h"
 #include "ginf_bp.h"
+#include "graphcut.h"
+#include "icm.h"
 
 namespace ginf {
 
@@ -253,5 +257,27 @@ template void decodeIcmAsync<float>(Grid<float> *, Matrix<int> *, int, Matrix<in
 template void decodeIcmAsync<int>(Grid<int> *, Matrix<int> *, int, Matrix<int> *);
 template void decodeMaxPrior<float>(Grid<float> *, Matrix<int> *);
 template void decodeMaxPrior<int>(Grid<int> *, Matrix<int> *);
+
+// ------------------------------------------------------------------ alpha-beta swap
+
+template <typename T>
+void decodeAlphaBeta(Grid<T> *grid, Matrix<int> *initial, int numIters, Matrix<int> *result) {
+    const int rc = ginf_alpha_beta_host(grid->getWidth(), grid->getHeight(), grid->getNumLabels(), dtype_of<T>(),
+                                        grid->getDataCosts()->data, grid->getSmCosts()->data, initial->data, numIters,
+                                        result->data);
+    if (rc != GINF_OK) {
+        fprintf(stderr, "ginf::decodeAlphaBeta failed (code %d): %s\n", rc, ginf_last_error());
+        exit(1);
+    }
+}
+
+template <typename T>
+void gpuDecodeAlphaBeta(Grid<T> *grid, Matrix<int> *initial, int numIters, Matrix<int> *result) {
+    decodeAlphaBeta(grid, initial, numIters, result);
+}
+
+// Grid<int> only, as the reference instantiates (src/graphcut.cc:281, src/gpu/graphcut.cu:250).
+template void decodeAlphaBeta<int>(Grid<int> *, Matrix<int> *, int, Matrix<int> *);
+template void gpuDecodeAlphaBeta<int>(Grid<int> *, Matrix<int> *, int, Matrix<int> *);
 
 }  // namespace ginf

@@ -26,9 +26,17 @@
 
 namespace {
 
+// Neighbour labels outside [0, L) (undefined in the reference) are clamped so
+// that the table reads stay inside the table.
+__device__ __forceinline__ int clamp_label(int f, int L) { return f < 0 ? 0 : (f >= L ? L - 1 : f); }
+
 template <typename T>
 __device__ __forceinline__ int icm_best(const T *__restrict__ data, const T *__restrict__ table, int64_t plane,
                                         int64_t node, int L, int fS, int fE, int fN, int fW) {
+    fS = clamp_label(fS, L);
+    fE = clamp_label(fE, L);
+    fN = clamp_label(fN, L);
+    fW = clamp_label(fW, L);
     const T *vS = table + (int64_t)L * fS;
     const T *vE = table + (int64_t)L * fE;
     const T *vN = table + (int64_t)L * fN;

@@ -226,6 +226,18 @@ int ginf_icm(int width, int height, int num_labels, int dtype, const void *data_
  * label of minimal data cost. */
 int ginf_max_prior(int width, int height, int num_labels, int dtype, const void *data_dev, int32_t *result_dev,
                    void *stream);
+/* Alpha-beta swap: decodeAlphaBeta / gpuDecodeAlphaBeta (src/include/graphcut.h:20-26,
+ * src/graphcut.cc:194-279), int costs only as in the reference.  For each
+ * label pair the swap graph of graphCutConstructGraph (graphcut.cc:9-46) is
+ * built from a fresh state, a parallel push-relabel finds a maximum preflow,
+ * nodes of the pair that can reach the sink take alpha and the rest beta,
+ * and the move is kept when getLabelingCost drops; up to `iters` rounds over
+ * all pairs, stopping after a round without a kept move.  *moves_out (may be
+ * NULL) receives the number of kept moves.  Synchronous on `stream`. */
+int ginf_alpha_beta(int width, int height, int num_labels, int dtype, const void *data_dev, const void *table_dev,
+                    const int32_t *initial_dev, int iters, int32_t *result_dev, int *moves_out, void *stream);
+int ginf_alpha_beta_host(int width, int height, int num_labels, int dtype, const void *data_host,
+                         const void *table_host, const int32_t *initial_host, int iters, int32_t *result_host);
 /* Synchronous host-pointer forms (used by include/icm.h and estimators.h). */
 int ginf_icm_host(int width, int height, int num_labels, int dtype, const void *data_host, const void *table_host,
                   const int32_t *initial_host, int iters, int mode, int32_t *result_host);

@@ -63,6 +63,9 @@ def _declare(L: ctypes.CDLL) -> None:
         for name in ("oracle_icm_sync", "oracle_icm_async"):
             getattr(L, f"{name}_{sfx}").argtypes = [P, P, i, i, i, P, i, P]
         getattr(L, f"oracle_max_prior_{sfx}").argtypes = [P, i, i, i, P]
+    L.oracle_ab_move.argtypes = [P, P, i, i, i, P, i, i]
+    L.oracle_alpha_beta.argtypes = [P, P, i, i, i, P, i, P]
+    L.oracle_alpha_beta.restype = i
     L.oracle_max_threads.restype = ctypes.c_int
 
 
@@ -169,6 +172,28 @@ def icm(data: np.ndarray, table: np.ndarray, initial: np.ndarray, iters: int, sy
     fn = getattr(lib(), f"oracle_icm_{'sync' if sync else 'async'}_{_sfx(data.dtype)}")
     fn(_ptr(data), _ptr(table), W, H, L, _ptr(init), iters, _ptr(out))
     return out
+
+
+def ab_move(data: np.ndarray, table: np.ndarray, labels: np.ndarray, alpha: int, beta: int) -> np.ndarray:
+    """One alpha-beta swap move (int costs), graph of src/graphcut.cc:9-46."""
+    data = np.ascontiguousarray(data, dtype=np.int32)
+    table = np.ascontiguousarray(table, dtype=np.int32)
+    f = np.ascontiguousarray(labels, dtype=np.int32).copy()
+    L, H, W = data.shape
+    lib().oracle_ab_move(_ptr(data), _ptr(table), W, H, L, _ptr(f), alpha, beta)
+    return f
+
+
+def alpha_beta(data: np.ndarray, table: np.ndarray, initial: np.ndarray, iters: int):
+    """decodeAlphaBeta (src/graphcut.cc:194-279) with fresh per-pair graphs.
+    Returns (labels, accepted moves)."""
+    data = np.ascontiguousarray(data, dtype=np.int32)
+    table = np.ascontiguousarray(table, dtype=np.int32)
+    init = np.ascontiguousarray(initial, dtype=np.int32)
+    L, H, W = data.shape
+    out = np.zeros((H, W), np.int32)
+    n = lib().oracle_alpha_beta(_ptr(data), _ptr(table), W, H, L, _ptr(init), iters, _ptr(out))
+    return out, n
 
 
 def max_prior(data: np.ndarray) -> np.ndarray:

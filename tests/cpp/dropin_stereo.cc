@@ -5,8 +5,10 @@
 // ICM refinement (include/icm.h).  Prints the labels, one row per line, and
 // the energies, for the tests to compare with the oracle.
 //
-//   dropin_stereo [W H iters levels]              synthetic pair (seed 1)
-//   dropin_stereo --pgm left.pgm right.pgm iters levels [icm_iters [async]]
+//   dropin_stereo [W H iters levels [refine_iters [sync|async|ab]]]   synthetic pair (seed 1)
+//   dropin_stereo --pgm left.pgm right.pgm iters levels [refine_iters [sync|async|ab]]
+// Refinement: ICM (gpuDecodeIcm / decodeIcmAsync) or alpha-beta swap
+// (decodeAlphaBeta, include/graphcut.h) started from the BP labels.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -16,6 +18,7 @@
 #include "core.h"
 #include "estimators.h"
 #include "ginf_bp.h"
+#include "graphcut.h"
 #include "icm.h"
 
 namespace {
@@ -40,7 +43,7 @@ bool read_pgm(const char *path, int &w, int &h, std::vector<uint8_t> &px) {
 int main(int argc, char **argv) {
     const int L = 16;
     int W = 96, H = 64, iters = 30, levels = 1, icm_iters = 0;
-    bool icm_async = false;
+    const char *refine = "sync";
     std::vector<uint8_t> left, right;
     if (argc > 1 && strcmp(argv[1], "--pgm") == 0) {
         int w2 = 0, h2 = 0;
@@ -51,12 +54,14 @@ int main(int argc, char **argv) {
         iters = atoi(argv[4]);
         levels = atoi(argv[5]);
         if (argc > 6) icm_iters = atoi(argv[6]);
-        if (argc > 7) icm_async = strcmp(argv[7], "async") == 0;
+        if (argc > 7) refine = argv[7];
     } else {
         if (argc > 1) W = atoi(argv[1]);
         if (argc > 2) H = atoi(argv[2]);
         if (argc > 3) iters = atoi(argv[3]);
         if (argc > 4) levels = atoi(argv[4]);
+        if (argc > 5) icm_iters = atoi(argv[5]);
+        if (argc > 6) refine = argv[6];
         left.resize((size_t)W * H);
         right.resize((size_t)W * H);
         if (ginf_gen_stereo(1, W, H, L, 8, 15, 20, left.data(), right.data(), nullptr, nullptr, nullptr) != 0) return 2;
@@ -76,7 +81,8 @@ int main(int argc, char **argv) {
     int e_icm = e_bp;
     if (icm_iters > 0) {
         ginf::Matrix<int> refined(2, grid.getWidth(), grid.getHeight());
-        if (icm_async) ginf::decodeIcmAsync(&grid, &result, icm_iters, &refined);
+        if (strcmp(refine, "async") == 0) ginf::decodeIcmAsync(&grid, &result, icm_iters, &refined);
+        else if (strcmp(refine, "ab") == 0) ginf::decodeAlphaBeta(&grid, &result, icm_iters, &refined);
         else ginf::gpuDecodeIcm(&grid, &result, icm_iters, &refined);
         result.copyFrom(&refined);
         e_icm = grid.getLabelingCost(&result);

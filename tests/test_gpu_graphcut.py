@@ -1,0 +1,64 @@
+"""GPU alpha-beta swap against the oracle: every kept move is the canonical
+optimal swap (the sink-reachable set of any maximum flow takes alpha), so the
+whole decode -- labels and the number of kept moves -- is reproduced bit for
+bit."""
+import numpy as np
+import pytest
+
+from gpu_util import torch_cuda, say
+from gridinfcuda_amd import _lib as G
+
+pytestmark = pytest.mark.gpu
+
+
+def sym_table(rng, L, kind):
+    a = np.arange(L)
+    if kind == "potts":
+        return (7 * (a[:, None] != a[None, :])).astype(np.int32)
+    if kind == "tl":
+        return np.minimum(3 * np.abs(a[:, None] - a[None, :]), 8).astype(np.int32)
+    t = np.triu(rng.integers(0, 12, size=(L, L)), 1)
+    return (t + t.T).astype(np.int32)
+
+
+@pytest.mark.parametrize("kind", ["potts", "tl", "random"])
+@pytest.mark.parametrize("L,H,W,iters", [(3, 3, 3, 2), (4, 9, 13, 3), (6, 24, 32, 4), (5, 2, 17, 2), (8, 31, 7, 2),
+                                         (1, 5, 5, 3), (4, 12, 12, 0)])
+def test_alpha_beta_matches_oracle(oracle_mod, kind, L, H, W, iters):
+    torch = torch_cuda()
+    say(f"alpha-beta {kind} L={L} {W}x{H} iters={iters}")
+    rng = np.random.default_rng(L * 1000 + H * 10 + W)
+    data = rng.integers(0, 25, size=(L, H, W)).astype(np.int32)
+    table = sym_table(rng, L, kind)
+    init = rng.integers(0, L, size=(H, W)).astype(np.int32)
+    want, want_moves = oracle_mod.alpha_beta(data, table, init, iters)
+    dd, td, idev = (torch.from_numpy(a).cuda() for a in (data, table, init))
+    out = torch.full((H, W), -1, dtype=torch.int32, device="cuda")
+    moves = G.alpha_beta(W, H, L, dd.data_ptr(), td.data_ptr(), idev.data_ptr(), iters, out.data_ptr(),
+                         torch.cuda.current_stream().cuda_stream)
+    np.testing.assert_array_equal(out.cpu().numpy(), want)
+    assert moves == want_moves
+    np.testing.assert_array_equal(G.alpha_beta_host(data, table, init, iters), want)
+
+
+def test_alpha_beta_refines_bp(oracle_mod):
+    """BP labels refined by alpha-beta swap: the energy never rises and the
+    result equals the oracle's."""
+    say("bp -> alpha-beta")
+    W, H, L = 40, 30, 6
+    g = G.gen_stereo(23, W, H, L, 4, L - 1, 20, want=("cost_i32",))
+    data = g["cost_i32"]
+    a = np.arange(L)
+    table = np.minimum(4 * np.abs(a[:, None] - a[None, :]), 12).astype(np.int32)
+    lab = G.hbp_host(data, table, G.SM_TRUNC_LINEAR, 2, 6)
+    want, _ = oracle_mod.alpha_beta(data, table, lab, 3)
+    got = G.alpha_beta_host(data, table, lab, 3)
+    np.testing.assert_array_equal(got, want)
+    assert oracle_mod.energy(data, table, got) <= oracle_mod.energy(data, table, lab)
+
+
+def test_alpha_beta_rejects_float():
+    data = np.zeros((3, 4, 4), np.float32)
+    with pytest.raises(G.GinfError) as e:
+        G.alpha_beta_host(data, np.zeros((3, 3), np.float32), np.zeros((4, 4), np.int32), 1)
+    assert e.value.code == 2
