@@ -15,6 +15,7 @@ Prints ONE JSON line.  Only one GPU: --gpus > 1 exits with a message.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import platform
@@ -124,6 +125,35 @@ def bench_batched(args) -> dict:
     }
 
 
+def hbm_probe(torch, G, mib_per_stream: int = 512) -> dict:
+    """Best bandwidth of the library's streaming probe (ginf_stream_probe):
+    5 read + 4 written float4 streams (the half-sweep's stream shape) and a
+    1 + 1 copy, each over a few grid sizes; HIP events on the launch stream."""
+    n = (mib_per_stream << 20) // 16
+    inp = torch.ones(5 * n * 4, device="cuda")
+    out = torch.empty(4 * n * 4, device="cuda")
+    s = torch.cuda.current_stream()
+    res = {}
+    for (r, w) in ((5, 4), (1, 1)):
+        best = 0.0
+        for blocks in (16384, 65536, 262144):
+            def go():
+                G.check(G.lib().ginf_stream_probe(r, w, ctypes.c_void_p(inp.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                                  n, blocks, ctypes.c_void_p(s.cuda_stream)))
+            go()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(s)
+            for _ in range(10):
+                go()
+            b.record(s)
+            b.synchronize()
+            ms = a.elapsed_time(b) / 10
+            best = max(best, (r + w) * n * 16 / ms / 1e6)
+        res[f"{r}r{w}w"] = round(best, 1)
+    del inp, out
+    return res
+
+
 def bench_warmstart(args) -> dict:
     """Learned warm start, 640x480, 64 labels, TL lambda=10 tau=3: per seeded
     evaluation instance (seeds 2000000...), plain HBP 5x10 (the judge) against
@@ -192,6 +222,7 @@ def main() -> int:
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 chunked, 2 quad (per-direction lanes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-api", action="store_true", help="skip the host-entry (PCIe-inclusive) timing")
+    ap.add_argument("--no-probe", action="store_true", help="skip the HBM streaming probe")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
     ap.add_argument("--workload", default="hbp", choices=["hbp", "batched", "warmstart", "warmstart-batch"],
                     help="hbp: 1920x1080x128 hierarchical BP (default); batched: 64 mixed MRFs; "
@@ -320,6 +351,11 @@ def main() -> int:
         },
         "energy_f64": energy,
     }
+    if not args.no_probe:
+        # Practical ceiling for the sweep's stream shape, measured now on this GPU.
+        probe = hbm_probe(torch, G)
+        out["roofline"]["probe_gbs"] = probe
+        out["roofline"]["frac_of_probe"] = round(achieved / probe["5r4w"], 4)
     if not args.no_api:
         # End to end through the synchronous host entry (ginf_hbp_host, what the
         # C++ templates call): pageable host costs in, labels out, PCIe both

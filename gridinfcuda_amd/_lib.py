@@ -31,7 +31,7 @@ EXPORTS = [
     "ginf_relu_bwd", "ginf_mse", "ginf_adam", "ginf_to_bf16", "ginf_minpool_costs", "ginf_block_mean_msgs",
     "ginf_stereo_costs", "ginf_batch_set_seeded", "ginf_batch_seed", "ginf_hbp_host_release",
     "ginf_icm", "ginf_max_prior", "ginf_icm_host", "ginf_max_prior_host", "ginf_alpha_beta",
-    "ginf_alpha_beta_host",
+    "ginf_alpha_beta_host", "ginf_stream_probe",
 ]
 
 
@@ -96,6 +96,7 @@ def lib() -> ctypes.CDLL:
         L.ginf_minpool_costs.argtypes = [VP, I, I, I, I, I, F, VP, VP]
         L.ginf_block_mean_msgs.argtypes = [VP, I, I, I, I, F, VP, VP]
         L.ginf_stereo_costs.argtypes = [VP, VP, I, I, I, I, VP, VP]
+        L.ginf_stream_probe.argtypes = [I, I, VP, VP, I64, I, VP]
         L.ginf_last_error.restype = ctypes.c_char_p
         L.ginf_version.restype = ctypes.c_char_p
         for name in EXPORTS:

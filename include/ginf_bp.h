@@ -244,6 +244,13 @@ int ginf_icm_host(int width, int height, int num_labels, int dtype, const void *
 int ginf_max_prior_host(int width, int height, int num_labels, int dtype, const void *data_host,
                         int32_t *result_host);
 
+/* HBM calibration probe (no reference counterpart): reads `reads` and
+ * writes `writes` float4 streams of n_vec4 elements each ((reads, writes) =
+ * (1, 1) or (5, 4), the half-sweep's stream shape); in_dev holds reads *
+ * n_vec4 float4, out_dev writes * n_vec4.  Stream-ordered. */
+int ginf_stream_probe(int reads, int writes, const void *in_dev, void *out_dev, int64_t n_vec4, int blocks,
+                      void *stream);
+
 const char *ginf_last_error(void);
 
 /* Library version string. */
