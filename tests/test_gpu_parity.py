@@ -155,6 +155,22 @@ def test_hbp_config2_reduced(oracle_mod):
     assert e_gpu == e_ref
 
 
+def test_hbp_config2_full_size(oracle_mod):
+    """The headline workload at full size -- 1920x1080, 128 labels, seed 2,
+    TL lambda=8 tau=4, 5 levels x 10 half-sweeps, FP32 mean normalisation --
+    against the oracle (16 OpenMP threads, bit-identical to one): labels,
+    level-0 messages and the FP64 energy all exact."""
+    say("config2 full 1920x1080x128 (oracle ~15 s on 16 host threads)")
+    data = stereo(2, 1920, 1080, 128, 64, 127, 40, np.float32)
+    table = oracle_mod.table_trunc_linear(128, 8, 32)
+    ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, 2, 5, 10, norm=0, threads=16, return_state=True)
+    lab, msgs, _ = run_plan(data, table, 2, 5, 10, norm=0, dump_level=0)
+    np.testing.assert_array_equal(lab, ref_lab)
+    assert np.array_equal(msgs, ref_msgs), f"{int((msgs != ref_msgs).sum())} messages differ"
+    del msgs, ref_msgs
+    assert gpu_energy(data, table, lab) == oracle_mod.energy_f64(data, table, ref_lab)
+
+
 def test_energy_kernel_matches_numpy(oracle_mod):
     rng = np.random.default_rng(3)
     for dt in ("int32", "float32"):
