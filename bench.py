@@ -125,6 +125,56 @@ def bench_batched(args) -> dict:
     }
 
 
+def bench_config1(args) -> dict:
+    """The reference's CPU-sized case: 128x96 grid, 16 labels, FP32 integer
+    costs (seed 1, 8 rectangles, disparities 0..15, truncation 20),
+    V = 10*min(|a-b|, 2), decodeBp with 10 half-sweeps, min normalisation.
+    GPU: device-resident plan runs, median of >= 100; CPU: the oracle on one
+    thread, median of >= 100 runs (the full workload, not a sample)."""
+    import torch
+    from gridinfcuda_amd import _lib as G
+    from oracle import oracle as O
+    W, H, L, iters = 128, 96, 16, 10
+    data = G.gen_stereo(1, W, H, L, 8, 15, 20)["cost_f32"]
+    table = O.table_trunc_linear(L, 10, 20)
+    plan = G.Plan(G.make_desc(W, H, L, G.F32, G.SM_TRUNC_LINEAR, 1, iters, G.NORM_MIN), table)
+    dd = torch.from_numpy(data).cuda()
+    lab = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    s = torch.cuda.current_stream()
+    runs = max(100, args.steps)
+    for _ in range(max(1, args.warmup)):
+        plan.run(dd.data_ptr(), lab.data_ptr(), s.cuda_stream)
+    ts = []
+    for _ in range(runs):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        plan.run(dd.data_ptr(), lab.data_ptr(), s.cuda_stream)
+        b.record(s)
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    gpu_ms = float(np.median(ts))
+    U = plan.updates_per_run()
+    cts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        ref = O.hbp(data, table, 2, 1, iters, norm=1)
+        cts.append((time.perf_counter() - t0) * 1000.0)
+    cpu_ms = float(np.median(cts))
+    same = bool(np.array_equal(ref, lab.cpu().numpy()))
+    return {
+        "metric": "Gnode-label msg updates/s", "value": round(U / gpu_ms / 1e6, 3), "unit": "Gnode-label msg updates/s",
+        "n_gpus": 1, "steps": runs, "warmup": args.warmup, "ms_per_step": round(gpu_ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded stereo generator, seed 1)",
+        "config": {"workload": "plain BP 128x96, 16 labels, V=10*min(|a-b|,2), 10 half-sweeps, min normalisation",
+                   "node_label_updates_per_step": U, "labels_equal_oracle": same},
+        "roofline": None,
+        "cpu_baseline": {"value": round(U / cpu_ms / 1e6, 5), "unit": "Gnode-label msg updates/s", "cores": 1,
+                         "kind": "port", "sample": f"the full workload, median of {runs} runs",
+                         "ms_per_step": round(cpu_ms, 4), "cpu_model": cpu_model()},
+    }
+
+
 def hbm_probe(torch, G, mib_per_stream: int = 512) -> dict:
     """Best bandwidth of the library's streaming probe (ginf_stream_probe):
     5 read + 4 written float4 streams (the half-sweep's stream shape) and a
@@ -224,10 +274,11 @@ def main() -> int:
     ap.add_argument("--no-api", action="store_true", help="skip the host-entry (PCIe-inclusive) timing")
     ap.add_argument("--no-probe", action="store_true", help="skip the HBM streaming probe")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
-    ap.add_argument("--workload", default="hbp", choices=["hbp", "batched", "warmstart", "warmstart-batch"],
+    ap.add_argument("--workload", default="hbp", choices=["hbp", "batched", "warmstart", "warmstart-batch", "config1"],
                     help="hbp: 1920x1080x128 hierarchical BP (default); batched: 64 mixed MRFs; "
                          "warmstart: 640x480x64 learned warm start vs plain HBP; "
-                         "warmstart-batch: 256-instance batched warm start with fallback")
+                         "warmstart-batch: 256-instance batched warm start with fallback; "
+                         "config1: 128x96x16 plain BP, GPU and one CPU core")
     ap.add_argument("--eval-instances", type=int, default=None,
                     help="warm-start workloads: instances evaluated (default 8 / 256)")
     ap.add_argument("--batch-iters", default="16", help="warmstart-batch: finest-level half-sweeps K (list)")
@@ -250,6 +301,9 @@ def main() -> int:
         return 1
     if args.workload == "batched":
         print(json.dumps(bench_batched(args)))
+        return 0
+    if args.workload == "config1":
+        print(json.dumps(bench_config1(args)))
         return 0
     if args.workload == "warmstart":
         args.eval_instances = args.eval_instances or 8

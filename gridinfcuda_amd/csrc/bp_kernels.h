@@ -100,4 +100,7 @@ hipError_t launch_zero_labels(int32_t *labels, int W, int H, int color, hipStrea
 hipError_t launch_energy(int dtype, const void *data, const void *table, const int32_t *labels, int W, int H, int L,
                          double *partials, int nblocks, double *out, hipStream_t st);
 
+// Records the message ginf_last_error() returns and passes `code` through.
+int set_last_error(int code, const char *msg);
+
 }  // namespace ginf

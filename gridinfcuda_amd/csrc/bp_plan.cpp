@@ -32,6 +32,16 @@ int set_error(int code, const char *fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+// Shared with the other translation units of the library (declared in bp_kernels.h).
+int ginf::set_last_error(int code, const char *msg) {
+    g_last_error = msg;
+    return code;
+}
+
+namespace {
+
 #define GINF_HIP_TRY(expr)                                                                                   \
     do {                                                                                                     \
         hipError_t e_ = (expr);                                                                              \

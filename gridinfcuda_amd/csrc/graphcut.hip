@@ -217,11 +217,13 @@ extern "C" {
 
 int ginf_alpha_beta(int width, int height, int num_labels, int dtype, const void *data_dev, const void *table_dev,
                     const int32_t *initial_dev, int iters, int32_t *result_dev, int *moves_out, void *stream) {
+    using ginf::set_last_error;
     if (!data_dev || !table_dev || !initial_dev || !result_dev || width < 1 || height < 1 || num_labels < 1 ||
         iters < 0)
-        return GINF_ERR_INVALID;
-    if (dtype != GINF_I32) return GINF_ERR_UNSUPPORTED;  // the reference instantiates int only
-    if ((int64_t)width * height > ((int64_t)1 << 28)) return GINF_ERR_UNSUPPORTED;
+        return set_last_error(GINF_ERR_INVALID, "NULL argument, bad sizes or iteration count");
+    if (dtype != GINF_I32)  // the reference instantiates int only
+        return set_last_error(GINF_ERR_UNSUPPORTED, "alpha-beta swap takes int costs only");
+    if ((int64_t)width * height > ((int64_t)1 << 28)) return set_last_error(GINF_ERR_UNSUPPORTED, "grid too large");
     const int64_t N = (int64_t)width * height;
     hipStream_t st = (hipStream_t)stream;
     Ctx c{st, (unsigned)std::min<int64_t>((N + 255) / 256, 4096), nullptr};
@@ -294,15 +296,15 @@ int ginf_alpha_beta(int width, int height, int num_labels, int dtype, const void
     if (c.host) (void)hipHostFree(c.host);
     if (ehost) (void)hipHostFree(ehost);
     if (moves_out) *moves_out = moves;
-    if (e == hipSuccess && bad_labels) return GINF_ERR_INVALID;
-    return e == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+    if (e == hipSuccess && bad_labels) return set_last_error(GINF_ERR_INVALID, "an initial label is outside [0, L)");
+    return e == hipSuccess ? GINF_OK : set_last_error(GINF_ERR_HIP, hipGetErrorString(e));
 }
 
 int ginf_alpha_beta_host(int width, int height, int num_labels, int dtype, const void *data_host,
                          const void *table_host, const int32_t *initial_host, int iters, int32_t *result_host) {
     if (!data_host || !table_host || !initial_host || !result_host || width < 1 || height < 1 || num_labels < 1)
         return GINF_ERR_INVALID;
-    if (dtype != GINF_I32) return GINF_ERR_UNSUPPORTED;
+    if (dtype != GINF_I32) return ginf::set_last_error(GINF_ERR_UNSUPPORTED, "alpha-beta swap takes int costs only");
     const size_t nd = (size_t)width * height * num_labels * 4, nt = (size_t)num_labels * num_labels * 4;
     const size_t nl = (size_t)width * height * sizeof(int32_t);
     void *dd = nullptr, *dt = nullptr, *di = nullptr, *dr = nullptr;

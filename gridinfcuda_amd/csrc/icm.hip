@@ -22,6 +22,7 @@
 
 #include <algorithm>
 
+#include "bp_kernels.h"
 #include "ginf_bp.h"
 
 namespace {
@@ -137,11 +138,15 @@ extern "C" {
 
 int ginf_icm(int width, int height, int num_labels, int dtype, const void *data_dev, const void *table_dev,
              const int32_t *initial_dev, int iters, int mode, int32_t *result_dev, void *stream) {
-    if (!data_dev || !table_dev || !initial_dev || !result_dev) return GINF_ERR_INVALID;
-    if (width < 1 || height < 1 || num_labels < 1 || iters < 0) return GINF_ERR_INVALID;
-    if (mode != GINF_ICM_SYNC && mode != GINF_ICM_ASYNC) return GINF_ERR_INVALID;
-    if ((int64_t)width * height > ((int64_t)1 << 30) || height - 2 > 65535) return GINF_ERR_UNSUPPORTED;
-    if (mode == GINF_ICM_ASYNC && iters > 65535) return GINF_ERR_UNSUPPORTED;
+    using ginf::set_last_error;
+    if (!data_dev || !table_dev || !initial_dev || !result_dev) return set_last_error(GINF_ERR_INVALID, "NULL argument");
+    if (width < 1 || height < 1 || num_labels < 1 || iters < 0)
+        return set_last_error(GINF_ERR_INVALID, "bad sizes or iteration count");
+    if (mode != GINF_ICM_SYNC && mode != GINF_ICM_ASYNC) return set_last_error(GINF_ERR_INVALID, "bad ICM mode");
+    if ((int64_t)width * height > ((int64_t)1 << 30) || height - 2 > 65535)
+        return set_last_error(GINF_ERR_UNSUPPORTED, "grid too large");
+    if (mode == GINF_ICM_ASYNC && iters > 65535)
+        return set_last_error(GINF_ERR_UNSUPPORTED, "more than 65535 asynchronous iterations");
     hipError_t e;
     if (dtype == GINF_F32)
         e = icm_t((const float *)data_dev, (const float *)table_dev, width, height, num_labels, initial_dev, iters,
@@ -150,8 +155,8 @@ int ginf_icm(int width, int height, int num_labels, int dtype, const void *data_
         e = icm_t((const int *)data_dev, (const int *)table_dev, width, height, num_labels, initial_dev, iters, mode,
                   result_dev, (hipStream_t)stream);
     else
-        return GINF_ERR_INVALID;
-    return e == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+        return set_last_error(GINF_ERR_INVALID, "dtype must be GINF_F32 or GINF_I32");
+    return e == hipSuccess ? GINF_OK : set_last_error(GINF_ERR_HIP, hipGetErrorString(e));
 }
 
 int ginf_max_prior(int width, int height, int num_labels, int dtype, const void *data_dev, int32_t *result_dev,

@@ -62,3 +62,13 @@ def test_alpha_beta_rejects_float():
     with pytest.raises(G.GinfError) as e:
         G.alpha_beta_host(data, np.zeros((3, 3), np.float32), np.zeros((4, 4), np.int32), 1)
     assert e.value.code == 2
+
+
+def test_alpha_beta_rejects_labels_out_of_range():
+    data = np.zeros((3, 5, 5), np.int32)
+    table = np.zeros((3, 3), np.int32)
+    init = np.zeros((5, 5), np.int32)
+    init[2, 2] = 7
+    with pytest.raises(G.GinfError) as e:
+        G.alpha_beta_host(data, table, init, 1)
+    assert e.value.code == 1

@@ -66,3 +66,13 @@ def test_icm_refines_bp_labels(oracle_mod):
     refined = G.icm_host(data, table, lab, 5, G.ICM_ASYNC)
     np.testing.assert_array_equal(refined, oracle_mod.icm(data, table, lab, 5, sync=False))
     assert oracle_mod.energy_f64(data, table, refined) <= oracle_mod.energy_f64(data, table, lab)
+
+
+def test_icm_invalid_arguments():
+    data = np.zeros((3, 5, 5), np.int32)
+    table = np.zeros((3, 3), np.int32)
+    init = np.zeros((5, 5), np.int32)
+    with pytest.raises(G.GinfError):
+        G.icm_host(data, table, init, -1, G.ICM_SYNC)
+    with pytest.raises(G.GinfError):
+        G.icm_host(data, table, init, 1, 7)
