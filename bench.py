@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E datasheet peak
 BYTES_PER_UPDATE = 36     # D (4) + 4 incoming (16) + 4 outgoing (16) FP32, per node-label
+                          # (34 with 16-bit data costs)
 
 
 def cpu_model() -> str:
@@ -273,6 +274,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-api", action="store_true", help="skip the host-entry (PCIe-inclusive) timing")
     ap.add_argument("--no-probe", action="store_true", help="skip the HBM streaming probe")
+    ap.add_argument("--data32", action="store_true",
+                    help="keep 32-bit data costs (no 16-bit storage even when the costs fit)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
     ap.add_argument("--workload", default="hbp", choices=["hbp", "batched", "warmstart", "warmstart-batch", "config1"],
                     help="hbp: 1920x1080x128 hierarchical BP (default); batched: 64 mixed MRFs; "
@@ -319,6 +322,8 @@ def main() -> int:
     desc = G.make_desc(W, H, L, G.F32, G.SM_TRUNC_LINEAR, args.levels, args.iters, args.norm, args.lanes,
                        args.kernel)
     plan = G.Plan(desc, table)
+    if args.data32:
+        plan.set_compact_data(False)
     data = torch.from_numpy(gen).cuda()
     del gen
     labels = torch.zeros((H, W), dtype=torch.int32, device="cuda")
@@ -351,14 +356,20 @@ def main() -> int:
     U = plan.updates_per_run()
     value = U / (ms_per_step / 1000.0) / 1e9
 
+    # With 16-bit data costs (integer costs in [0, 65535], the plan's default for
+    # the quad kernel) an update reads 2 B of data cost instead of 4.
+    compact = plan.compact_data()
+    bytes_per_update = BYTES_PER_UPDATE - (2 if compact else 0)
     avg_launch_ms = sweep_ms / max(1, launches)
-    bytes_per_launch = BYTES_PER_UPDATE * sweep_updates / max(1, launches)
+    bytes_per_launch = bytes_per_update * sweep_updates / max(1, launches)
     achieved = bytes_per_launch / (avg_launch_ms / 1000.0) / 1e9
+    quad = args.kernel != 1
+    kname = ("k_sweep_quad16" if compact else "k_sweep_quad") if quad else "k_sweep_linear"
 
     # HBM bytes per steady half-sweep launch, measured with rocprofv3 FETCH_SIZE /
     # WRITE_SIZE (scaled by a 1 GiB copy) on this workload; see the profile summary.
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "sweep_quad_1920x1080x128_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"{'sweep_quad16' if compact else 'sweep_quad'}_1920x1080x128_traffic.json")
     default_shape = (W, H, L, args.levels, args.iters, args.norm, args.kernel, args.lanes) == (1920, 1080, 128, 5, 10, 0, 0, 0)
     if default_shape and os.path.exists(tpath):
         with open(tpath) as f:
@@ -396,9 +407,9 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "k_sweep_quad steady-state instance (half-sweep)" if args.kernel != 1 else
-                      "k_sweep_linear steady-state instance (half-sweep)",
-            "bytes_per_update": BYTES_PER_UPDATE,
+            "kernel": f"{kname} steady-state instance (half-sweep)",
+            "bytes_per_update": bytes_per_update,
+            "data_costs": "16-bit" if compact else "32-bit",
             "avg_launch_ms": round(avg_launch_ms, 5),
             "launches_timed": launches,
             "algorithmic_bytes_per_launch": bytes_per_launch,

@@ -31,7 +31,7 @@ EXPORTS = [
     "ginf_relu_bwd", "ginf_mse", "ginf_adam", "ginf_to_bf16", "ginf_minpool_costs", "ginf_block_mean_msgs",
     "ginf_stereo_costs", "ginf_batch_set_seeded", "ginf_batch_seed", "ginf_hbp_host_release",
     "ginf_icm", "ginf_max_prior", "ginf_icm_host", "ginf_max_prior_host", "ginf_alpha_beta",
-    "ginf_alpha_beta_host", "ginf_stream_probe",
+    "ginf_alpha_beta_host", "ginf_stream_probe", "ginf_bp_plan_set_compact_data", "ginf_bp_plan_compact_data",
 ]
 
 
@@ -97,6 +97,8 @@ def lib() -> ctypes.CDLL:
         L.ginf_block_mean_msgs.argtypes = [VP, I, I, I, I, F, VP, VP]
         L.ginf_stereo_costs.argtypes = [VP, VP, I, I, I, I, VP, VP]
         L.ginf_stream_probe.argtypes = [I, I, VP, VP, I64, I, VP]
+        L.ginf_bp_plan_set_compact_data.argtypes = [VP, I]
+        L.ginf_bp_plan_compact_data.argtypes = [VP, P(I)]
         L.ginf_last_error.restype = ctypes.c_char_p
         L.ginf_version.restype = ctypes.c_char_p
         for name in EXPORTS:
@@ -180,6 +182,15 @@ class Plan:
     def block_mean_msgs(self, level: int, f: int, scale: float, out_ptr: int, stream: int = 0) -> None:
         check(lib().ginf_bp_plan_block_mean_msgs(self._h, level, f, scale, ctypes.c_void_p(out_ptr),
                                                  ctypes.c_void_p(stream)))
+
+    def set_compact_data(self, enable: bool) -> None:
+        check(lib().ginf_bp_plan_set_compact_data(self._h, 1 if enable else 0))
+
+    def compact_data(self) -> bool:
+        """Whether the last run stored the data costs as 16-bit integers."""
+        v = ctypes.c_int()
+        check(lib().ginf_bp_plan_compact_data(self._h, ctypes.byref(v)))
+        return bool(v.value)
 
     def set_sweep_timing(self, enable: bool) -> None:
         check(lib().ginf_bp_plan_set_sweep_timing(self._h, 1 if enable else 0))

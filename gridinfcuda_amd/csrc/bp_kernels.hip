@@ -3,6 +3,8 @@
 // the half-sweep dispatch (kernels in sweep_*.hip).  Layout: see bp_layout.h.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bp_kernels.h"
 
 namespace ginf {
@@ -34,10 +36,43 @@ __global__ __launch_bounds__(256) void k_relayout(const T *__restrict__ src, int
 // when `coarse` is set, the level-1 costs of coarse row Y: the children
 // (2X,2Y), (2X+1,2Y), (2X,2Y+1), (2X+1,2Y+1) added in that order from 0
 // (src/bp.cc:100-117).  Padding labels (L <= l < C*G) get cost 0.
-template <typename T, int TW>
+// Storage of a data cost: T itself, or a 16-bit unsigned integer when S is
+// uint16_t.  to_store records in *bad when a value has no exact 16-bit form
+// (not an integer in [0, 65535], or -0), and the plan then redoes the level
+// data in T.
+template <typename T>
+__device__ __forceinline__ bool fits_u16(T v) {
+    if constexpr (std::is_same<T, float>::value)
+        return v >= T(0) && v <= T(65535) && v == __builtin_truncf(v) && !__builtin_signbit(v);
+    else
+        return v >= T(0) && v <= T(65535);
+}
+
+template <typename S, typename T>
+__device__ __forceinline__ S to_store(T v, bool &bad) {
+    if constexpr (sizeof(S) == 2) {
+        bad |= !fits_u16(v);
+        return (S)(uint32_t)(fits_u16(v) ? v : T(0));
+    } else {
+        return v;
+    }
+}
+
+template <typename S, typename T>
+__device__ __forceinline__ T from_store(S v) {
+    if constexpr (sizeof(S) == 2) return T((uint32_t)v);
+    else return v;
+}
+
+__device__ __forceinline__ void report_bad(bool bad, int *flag) {
+    if (bad && flag) atomicOr(flag, 1);
+}
+
+template <typename T, typename S, int TW>
 __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src, int W, int H, int L, StateLayout ly,
-                                                      LevelGeom g0, T *d00, T *d01, int coarse, LevelGeom g1, T *d10,
-                                                      T *d11) {
+                                                      LevelGeom g0, S *d00, S *d01, int coarse, LevelGeom g1, S *d10,
+                                                      S *d11, int *flag) {
+    bool bad = false;
     extern __shared__ unsigned char smem_raw[];
     constexpr int TP = TW + 1;  // padded row: label-strided LDS reads hit distinct banks
     T *tile = reinterpret_cast<T *>(smem_raw);  // [2][LP][TP]
@@ -86,15 +121,18 @@ __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src,
                 const int x = x0 + xl;
                 if (y >= H || x >= W) continue;
                 const int64_t slot = (int64_t)y * g0.whp + (x >> 1);
-                T *dst = c ? d01 : d00;
+                S *dst = c ? d01 : d00;
                 for (int i = sub; i < C; i += ni) {
                     const int l = ly.interleave ? (i << gs) + q : q * C + i;
-                    dst[((int64_t)i * g0.nplane + slot) * G + q] = tile[(rr * LP + l) * TP + xl];
+                    dst[((int64_t)i * g0.nplane + slot) * G + q] = to_store<S>(tile[(rr * LP + l) * TP + xl], bad);
                 }
             }
         }
     }
-    if (!coarse) return;
+    if (!coarse) {
+        report_bad(bad, flag);
+        return;
+    }
     // level 1: coarse row Y, QS slots per colour
     constexpr int QS = TW / 4;
     {
@@ -102,14 +140,17 @@ __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src,
         const int ni = 256 / per > 0 ? 256 / per : 1;
         const int sub = tid / per, r = tid % per;
         const int q = r & (G - 1), k = r >> gs;
-        if (sub >= ni) return;
+        if (sub >= ni) {
+            report_bad(bad, flag);
+            return;
+        }
         for (int c = 0; c < 2; ++c) {
             const int Xl = 2 * k + ((Y + c) & 1);
             const int X = x0 / 2 + Xl;
             if (X >= g1.w) continue;
             const bool ex = 2 * X + 1 < W, ey = 2 * Y + 1 < H;
             const int64_t slot = (int64_t)Y * g1.whp + (X >> 1);
-            T *dst = c ? d11 : d10;
+            S *dst = c ? d11 : d10;
             for (int i = sub; i < C; i += ni) {
                 const int l = ly.interleave ? (i << gs) + q : q * C + i;
                 const T *t0 = tile + l * TP;
@@ -119,19 +160,20 @@ __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src,
                 if (ex) acc = acc + t0[2 * Xl + 1];
                 if (ey) acc = acc + t1[2 * Xl];
                 if (ex && ey) acc = acc + t1[2 * Xl + 1];
-                dst[((int64_t)i * g1.nplane + slot) * G + q] = acc;
+                dst[((int64_t)i * g1.nplane + slot) * G + q] = to_store<S>(acc, bad);
             }
         }
     }
+    report_bad(bad, flag);
 }
 
 // Data-cost pyramid level (src/bp.cc:100-117): parent (X,Y) sums its children
 // (2X,2Y), (2X+1,2Y), (2X,2Y+1), (2X+1,2Y+1) in that order, starting from 0.
 // Children (2X,2Y) and (2X+1,2Y+1) have colour 0, the other two colour 1, and
 // all four sit in slot column X of their rows.
-template <typename T>
-__global__ __launch_bounds__(256) void k_pyramid(const T *f0, const T *f1, LevelGeom fg, T *c0, T *c1,
-                                                 LevelGeom cg, int C, int G) {
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_pyramid(const S *f0, const S *f1, LevelGeom fg, S *c0, S *c1,
+                                                 LevelGeom cg, int C, int G, int *flag) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int node = gid / G, q = gid - (gid / G) * G;
     const int X = node % cg.w, Y = node / cg.w;
@@ -139,17 +181,19 @@ __global__ __launch_bounds__(256) void k_pyramid(const T *f0, const T *f1, Level
     const bool ex = 2 * X + 1 < fg.w, ey = 2 * Y + 1 < fg.h;
     const int64_t s00 = (int64_t)(2 * Y) * fg.whp + X;
     const int64_t s01 = (int64_t)(2 * Y + 1) * fg.whp + X;
-    T *dst = ((X + Y) & 1) ? c1 : c0;
+    S *dst = ((X + Y) & 1) ? c1 : c0;
     const int64_t cs = (int64_t)Y * cg.whp + (X >> 1);
+    bool bad = false;
     for (int i = 0; i < C; ++i) {
         const int64_t fo = (int64_t)i * fg.nplane;
         T acc = T(0);
-        acc = acc + f0[(fo + s00) * G + q];
-        if (ex) acc = acc + f1[(fo + s00) * G + q];
-        if (ey) acc = acc + f1[(fo + s01) * G + q];
-        if (ex && ey) acc = acc + f0[(fo + s01) * G + q];
-        dst[((int64_t)i * cg.nplane + cs) * G + q] = acc;
+        acc = acc + from_store<S, T>(f0[(fo + s00) * G + q]);
+        if (ex) acc = acc + from_store<S, T>(f1[(fo + s00) * G + q]);
+        if (ey) acc = acc + from_store<S, T>(f1[(fo + s01) * G + q]);
+        if (ex && ey) acc = acc + from_store<S, T>(f0[(fo + s01) * G + q]);
+        dst[((int64_t)i * cg.nplane + cs) * G + q] = to_store<S>(acc, bad);
     }
+    report_bad(bad, flag);
 }
 
 // Explicit outbox upsample (src/bp.cc:129-139), every fine node including borders.
@@ -232,8 +276,11 @@ __global__ __launch_bounds__(256) void k_energy_final(const double *partials, in
 }  // namespace
 
 hipError_t launch_sweep(int dtype, const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
-                        hipStream_t st) {
-    if (sh.quad) return launch_sweep_quad(dtype, sh, a, zeroin, labels, st);
+                        hipStream_t st, bool data16) {
+    if (data16 && !sh.quad) return hipErrorInvalidValue;
+    if (sh.quad)
+        return data16 ? launch_sweep_quad16(dtype, sh, a, zeroin, labels, st)
+                      : launch_sweep_quad(dtype, sh, a, zeroin, labels, st);
     if (sh.general) return launch_sweep_general(dtype, sh, a, zeroin, labels, st);
     return dtype == 0 ? launch_sweep_chunked_f32(sh, a, zeroin, labels, st)
                       : launch_sweep_chunked_i32(sh, a, zeroin, labels, st);
@@ -252,31 +299,36 @@ hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, cons
 
 hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
                                const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
-                               void *d11, hipStream_t st) {
+                               void *d11, hipStream_t st, bool data16, int *flag) {
     const int LP = ly.C * ly.G;
     const int tw = LP > 64 ? 32 : 64;
     const size_t lds = (size_t)2 * LP * (tw + 1) * 4;
     if (lds > 160 * 1024 || (tw / 2) * ly.G > 256) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((W + tw - 1) / tw), (unsigned)((H + 1) / 2)), block(256);
-#define GINF_RP(TT, TWW)                                                                                            \
-    hipLaunchKernelGGL((k_relayout_pyr<TT, TWW>), grid, block, lds, st, (const TT *)src, W, H, L, ly, g0, (TT *)d00, \
-                       (TT *)d01, coarse ? 1 : 0, g1, (TT *)d10, (TT *)d11)
-    if (dtype == 0) { if (tw == 32) GINF_RP(float, 32); else GINF_RP(float, 64); }
-    else { if (tw == 32) GINF_RP(int, 32); else GINF_RP(int, 64); }
+#define GINF_RP(TT, SS, TWW)                                                                                     \
+    hipLaunchKernelGGL((k_relayout_pyr<TT, SS, TWW>), grid, block, lds, st, (const TT *)src, W, H, L, ly, g0,      \
+                       (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag)
+    if (data16) {
+        if (dtype == 0) { if (tw == 32) GINF_RP(float, uint16_t, 32); else GINF_RP(float, uint16_t, 64); }
+        else { if (tw == 32) GINF_RP(int, uint16_t, 32); else GINF_RP(int, uint16_t, 64); }
+    } else {
+        if (dtype == 0) { if (tw == 32) GINF_RP(float, float, 32); else GINF_RP(float, float, 64); }
+        else { if (tw == 32) GINF_RP(int, int, 32); else GINF_RP(int, int, 64); }
+    }
 #undef GINF_RP
     return hipGetLastError();
 }
 
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
-                          const LevelGeom &cg, int C, int G, hipStream_t st) {
+                          const LevelGeom &cg, int C, int G, hipStream_t st, bool data16, int *flag) {
     const int64_t threads = (int64_t)cg.w * cg.h * G;
     const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
-    if (dtype == 0)
-        hipLaunchKernelGGL(k_pyramid<float>, grid, block, 0, st, (const float *)f0, (const float *)f1, fg,
-                           (float *)c0, (float *)c1, cg, C, G);
-    else
-        hipLaunchKernelGGL(k_pyramid<int>, grid, block, 0, st, (const int *)f0, (const int *)f1, fg, (int *)c0,
-                           (int *)c1, cg, C, G);
+#define GINF_PY(TT, SS)                                                                                            \
+    hipLaunchKernelGGL((k_pyramid<TT, SS>), grid, block, 0, st, (const SS *)f0, (const SS *)f1, fg, (SS *)c0,      \
+                       (SS *)c1, cg, C, G, flag)
+    if (data16) { if (dtype == 0) GINF_PY(float, uint16_t); else GINF_PY(int, uint16_t); }
+    else { if (dtype == 0) GINF_PY(float, float); else GINF_PY(int, int); }
+#undef GINF_PY
     return hipGetLastError();
 }
 

@@ -60,7 +60,7 @@ struct SweepArgs {
     uint32_t data_bytes;     // bytes of the data-cost array of the moving colour
     uint32_t in_span;        // bytes addressable from in_base
     int lmax;                // label capacity of the quad kernel instance (batched launches)
-    int pad_;
+    uint32_t data_plane_bytes;  // bytes between data-cost planes (half of plane_bytes for 16-bit costs)
 };
 
 }  // namespace ginf

@@ -112,6 +112,13 @@ struct ginf_bp_plan_s {
     float s_f = 0, tau_f = 0;
     int s_i = 0, tau_i = 0;
     int device = 0;
+    // 16-bit data costs (quad kernel): on by default; a run uses them when the
+    // relayout and pyramid find every cost of every level to be an integer in
+    // [0, 65535] (flag_dev stays 0), else it redoes the level data in T.
+    bool compact = true;
+    bool compact_last = false;
+    int *flag_dev = nullptr;
+    int *flag_host = nullptr;  // pinned
     // sweep timing (profiling hook)
     bool timing = false;
     struct Timed { hipEvent_t beg, end; double updates; };
@@ -205,13 +212,32 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
     const StateLayout ly = state_layout(p->shape, d.num_labels);
     const int dt = d.dtype, C = ly.C, G = ly.G, T = p->levels;
     const int W = d.width, H = d.height, L = d.num_labels;
-    // Level 0 (and level 1) costs in one pass over the caller's array.
+    // Level 0 (and level 1) costs in one pass over the caller's array, then
+    // the coarser levels.  With 16-bit storage the host reads the fit flag
+    // once (one stream synchronisation per run) and falls back to T storage
+    // when a cost has no exact 16-bit form; both forms give the same values.
     const bool two = T > 1;
-    GINF_HIP_TRY(launch_relayout_pyr(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0], p->lv[0].data[1], two,
-                                     p->geom[two ? 1 : 0], p->lv[two ? 1 : 0].data[0], p->lv[two ? 1 : 0].data[1], st));
-    for (int t = 2; t <= start; ++t)
-        GINF_HIP_TRY(launch_pyramid(dt, p->lv[t - 1].data[0], p->lv[t - 1].data[1], p->geom[t - 1], p->lv[t].data[0],
-                                    p->lv[t].data[1], p->geom[t], C, G, st));
+    auto prepare = [&](bool d16) -> int {
+        int *flag = d16 ? p->flag_dev : nullptr;
+        GINF_HIP_TRY(launch_relayout_pyr(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0], p->lv[0].data[1],
+                                         two, p->geom[two ? 1 : 0], p->lv[two ? 1 : 0].data[0],
+                                         p->lv[two ? 1 : 0].data[1], st, d16, flag));
+        for (int t = 2; t <= start; ++t)
+            GINF_HIP_TRY(launch_pyramid(dt, p->lv[t - 1].data[0], p->lv[t - 1].data[1], p->geom[t - 1],
+                                        p->lv[t].data[0], p->lv[t].data[1], p->geom[t], C, G, st, d16, flag));
+        return GINF_OK;
+    };
+    bool d16 = p->compact && p->shape.quad && p->flag_dev && p->flag_host;
+    if (d16) {
+        GINF_HIP_TRY(launch_fill_u32((uint32_t *)p->flag_dev, 1, 0u, st));
+        if (int rc = prepare(true)) return rc;
+        GINF_HIP_TRY(hipMemcpyAsync(p->flag_host, p->flag_dev, sizeof(int), hipMemcpyDeviceToHost, st));
+        GINF_HIP_TRY(hipStreamSynchronize(st));
+        if (*p->flag_host) d16 = false;
+    }
+    if (!d16)
+        if (int rc = prepare(false)) return rc;
+    p->compact_last = d16;
     for (int t = start; t >= 0; --t) {
         const LevelGeom &g = p->geom[t];
         const int64_t per = (int64_t)C * g.nplane * G;
@@ -267,8 +293,9 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
             a.table = p->table_dev;
             // byte view for the quad kernel
             a.arr_bytes = (uint32_t)p->arr_bytes[t];
-            a.data_bytes = (uint32_t)p->arr_bytes[t];
+            a.data_bytes = (uint32_t)(d16 ? p->arr_bytes[t] / 2 : p->arr_bytes[t]);
             a.plane_bytes = (uint32_t)(g.nplane * G * 4);
+            a.data_plane_bytes = d16 ? a.plane_bytes / 2 : a.plane_bytes;
             a.out_base = p->lv[t].msgs[color][0];
             if (src_level == t) {
                 a.in_base = p->lv[t].msgs[1 - color][0];
@@ -291,7 +318,7 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
                 tm.updates = (double)interior_of_color(g.w, g.h, color) * L;
                 if (tm.beg) GINF_HIP_TRY(hipEventRecord(tm.beg, st));
             }
-            GINF_HIP_TRY(launch_sweep(dt, p->shape, a, zeroin, labels, st));
+            GINF_HIP_TRY(launch_sweep(dt, p->shape, a, zeroin, labels, st, d16));
             if (tm.beg && tm.end) {
                 GINF_HIP_TRY(hipEventRecord(tm.end, st));
                 p->timed.push_back(tm);
@@ -408,7 +435,13 @@ int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, gin
         p->table_dev = base + total - round_up((size_t)L * L * es, 256);
         e = hipMemcpy(p->table_dev, sm_table_host, (size_t)L * L * es, hipMemcpyHostToDevice);
     }
+    if (e == hipSuccess && p->shape.quad) {
+        e = hipMalloc((void **)&p->flag_dev, sizeof(int));
+        if (e == hipSuccess) e = hipHostMalloc((void **)&p->flag_host, sizeof(int), hipHostMallocDefault);
+    }
     if (e != hipSuccess) {
+        if (p->flag_dev) hipFree(p->flag_dev);
+        if (p->flag_host) hipHostFree(p->flag_host);
         hipFree(p->pool);
         delete p;
         return set_error(GINF_ERR_HIP, "plan set-up failed: %s", hipGetErrorString(e));
@@ -422,6 +455,8 @@ int ginf_bp_plan_destroy(ginf_bp_plan plan) {
     for (auto &t : plan->timed) { if (t.beg) hipEventDestroy(t.beg); if (t.end) hipEventDestroy(t.end); }
     for (auto e : plan->spare) hipEventDestroy(e);
     if (plan->pool) hipFree(plan->pool);
+    if (plan->flag_dev) hipFree(plan->flag_dev);
+    if (plan->flag_host) hipHostFree(plan->flag_host);
     delete plan;
     return GINF_OK;
 }
@@ -429,6 +464,18 @@ int ginf_bp_plan_destroy(ginf_bp_plan plan) {
 int ginf_bp_plan_workspace_bytes(ginf_bp_plan plan, size_t *bytes) {
     if (!plan || !bytes) return set_error(GINF_ERR_INVALID, "NULL argument");
     *bytes = plan->pool_bytes;
+    return GINF_OK;
+}
+
+int ginf_bp_plan_set_compact_data(ginf_bp_plan plan, int enable) {
+    if (!plan) return set_error(GINF_ERR_INVALID, "NULL plan");
+    plan->compact = enable != 0;
+    return GINF_OK;
+}
+
+int ginf_bp_plan_compact_data(ginf_bp_plan plan, int *last_run_compact) {
+    if (!plan || !last_run_compact) return set_error(GINF_ERR_INVALID, "NULL argument");
+    *last_run_compact = plan->compact_last ? 1 : 0;
     return GINF_OK;
 }
 
@@ -557,10 +604,23 @@ int ginf_bp_plan_dump_data(ginf_bp_plan p, int level, void *data_host) {
     const int C = ly.C, G = ly.G;
     const size_t n = (size_t)C * g.nplane * G;
     std::vector<uint32_t> buf(n);
+    std::vector<uint16_t> buf16(p->compact_last ? n : 0);
     uint32_t *dst = (uint32_t *)data_host;
     GINF_HIP_TRY(hipDeviceSynchronize());
     for (int c = 0; c < 2; ++c) {
-        GINF_HIP_TRY(hipMemcpy(buf.data(), p->lv[level].data[c], n * 4, hipMemcpyDeviceToHost));
+        if (p->compact_last) {  // 16-bit storage of the last run: widen to T
+            GINF_HIP_TRY(hipMemcpy(buf16.data(), p->lv[level].data[c], n * 2, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < n; ++k) {
+                if (p->desc.dtype == GINF_F32) {
+                    const float f = (float)buf16[k];
+                    std::memcpy(&buf[k], &f, 4);
+                } else {
+                    buf[k] = buf16[k];
+                }
+            }
+        } else {
+            GINF_HIP_TRY(hipMemcpy(buf.data(), p->lv[level].data[c], n * 4, hipMemcpyDeviceToHost));
+        }
         for (int y = 0; y < g.h; ++y)
             for (int x = 0; x < g.w; ++x) {
                 if (((x + y) & 1) != c) continue;
@@ -814,7 +874,7 @@ int ginf_batch_create(int n, const ginf_bp_desc *descs, const void *const *table
             a.s_f = p->s_f; a.tau_f = p->tau_f; a.s_i = p->s_i; a.tau_i = p->tau_i;
             a.table = p->table_dev;
             a.arr_bytes = a.data_bytes = a.in_arr_bytes = (uint32_t)p->arr_bytes[0];
-            a.plane_bytes = a.in_plane_bytes = (uint32_t)(g.nplane * ly.G * 4);
+            a.plane_bytes = a.in_plane_bytes = a.data_plane_bytes = (uint32_t)(g.nplane * ly.G * 4);
             a.out_base = p->lv[0].msgs[c][0];
             a.in_base = p->lv[0].msgs[1 - c][0];
             a.in_span = (uint32_t)(4 * p->arr_bytes[0]);

@@ -22,13 +22,16 @@ def say(*args):
 
 def run_plan(data: np.ndarray, table: np.ndarray, model: int, levels: int, iters: int, norm: int = 0,
              lanes: int = 0, labels_init: np.ndarray | None = None, dump_level: int | None = None,
-             runs: int = 1, kernel: int = 0):
-    """HBP on the GPU via ginf_bp_plan_*; returns (labels, msgs-or-None, plan)."""
+             runs: int = 1, kernel: int = 0, compact: bool | None = None):
+    """HBP on the GPU via ginf_bp_plan_*; returns (labels, msgs-or-None, plan).
+    compact: None keeps the plan's default (16-bit data costs when they fit)."""
     torch = torch_cuda()
     data = np.ascontiguousarray(data)
     L, H, W = data.shape
     desc = G.make_desc(W, H, L, G.dtype_code(data.dtype), model, levels, iters, norm, lanes, kernel)
     plan = G.Plan(desc, np.ascontiguousarray(table, dtype=data.dtype))
+    if compact is not None:
+        plan.set_compact_data(compact)
     dd = torch.from_numpy(data).cuda()
     init = np.zeros((H, W), np.int32) if labels_init is None else labels_init.astype(np.int32)
     out = None

@@ -37,14 +37,17 @@ def stereo(seed, W, H, L, rects, dmax, trunc, dtype):
 # ---------------------------------------------------------------- config 1 shape
 
 @pytest.mark.parametrize("dtype,norm", [(np.int32, 0), (np.float32, 1), (np.float32, 0), (np.int32, 1)])
-@pytest.mark.parametrize("lanes,kernel", [(0, 0), (1, 1), (2, 1), (4, 1), (0, 2)])
-def test_config1_bit_exact(oracle_mod, dtype, norm, lanes, kernel):
+@pytest.mark.parametrize("lanes,kernel,compact", [(0, 0, None), (1, 1, None), (2, 1, None), (4, 1, None), (0, 2, True),
+                                                  (0, 2, False)])
+def test_config1_bit_exact(oracle_mod, dtype, norm, lanes, kernel, compact):
     """128x96, 16 labels, TL (10,20), 10 half-sweeps, stereo seed 1."""
-    say(f"config1 dtype={np.dtype(dtype).name} norm={norm} lanes={lanes} kernel={kernel}")
+    say(f"config1 dtype={np.dtype(dtype).name} norm={norm} lanes={lanes} kernel={kernel} compact={compact}")
     data = stereo(1, 128, 96, 16, 8, 15, 20, dtype)
     table = oracle_mod.table_trunc_linear(16, 10, 20, dtype)
     ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, 2, 1, 10, norm=norm, return_state=True)
-    lab, msgs, _ = run_plan(data, table, 2, 1, 10, norm=norm, lanes=lanes, dump_level=0, runs=2, kernel=kernel)
+    lab, msgs, plan = run_plan(data, table, 2, 1, 10, norm=norm, lanes=lanes, dump_level=0, runs=2, kernel=kernel,
+                               compact=compact)
+    assert plan.compact_data() == (kernel == 2 and compact is not False)
     exact = np.dtype(dtype) == np.int32 or norm == 1 or lanes in (0, 1) or kernel == 2
     if exact:
         np.testing.assert_array_equal(lab, ref_lab)
@@ -91,8 +94,9 @@ def test_linear_bit_exact(oracle_mod, H, W, L, lanes, levels, iters, kind, dt, n
     table = table_for(oracle_mod, kind, L, dtype)
     init = rng.integers(0, L, size=(H, W)).astype(np.int32)
     try:
+        # quad cases alternate between 16-bit and T storage of the data costs
         lab, msgs, _ = run_plan(data, table, SM[kind], levels, iters, norm=norm, lanes=lanes, labels_init=init,
-                                dump_level=0, kernel=kernel)
+                                dump_level=0, kernel=kernel, compact=bool((H + W + L) & 1) if kernel == 2 else None)
     except G.GinfError as e:
         assert e.code == 2, e  # unsupported (lanes, L) pair
         pytest.skip(str(e))
@@ -164,10 +168,12 @@ def test_hbp_config2_full_size(oracle_mod):
     data = stereo(2, 1920, 1080, 128, 64, 127, 40, np.float32)
     table = oracle_mod.table_trunc_linear(128, 8, 32)
     ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, 2, 5, 10, norm=0, threads=16, return_state=True)
-    lab, msgs, _ = run_plan(data, table, 2, 5, 10, norm=0, dump_level=0)
-    np.testing.assert_array_equal(lab, ref_lab)
-    assert np.array_equal(msgs, ref_msgs), f"{int((msgs != ref_msgs).sum())} messages differ"
-    del msgs, ref_msgs
+    for compact in (True, False):  # 16-bit and 32-bit storage of the data costs
+        lab, msgs, plan = run_plan(data, table, 2, 5, 10, norm=0, dump_level=0, compact=compact)
+        assert plan.compact_data() == compact
+        np.testing.assert_array_equal(lab, ref_lab)
+        assert np.array_equal(msgs, ref_msgs), f"compact={compact}: {int((msgs != ref_msgs).sum())} messages differ"
+        del msgs, plan
     assert gpu_energy(data, table, lab) == oracle_mod.energy_f64(data, table, ref_lab)
 
 
@@ -193,8 +199,9 @@ def test_invalid_arguments_rejected():
     assert e.value.code == 1
 
 
+@pytest.mark.parametrize("compact", [True, False])
 @pytest.mark.parametrize("iters", [2, 3])
-def test_labelling_sweeps_repeatable(oracle_mod, iters):
+def test_labelling_sweeps_repeatable(oracle_mod, iters, compact):
     """Regression: the labelling half-sweeps at a size with many waves in
     flight give the oracle's messages on every repetition (an earlier build
     corrupted them intermittently through a reused scalar offset)."""
@@ -203,7 +210,7 @@ def test_labelling_sweeps_repeatable(oracle_mod, iters):
     table = oracle_mod.table_trunc_linear(128, 8, 32)
     ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, 2, 1, iters, norm=0, return_state=True, threads=16)
     for rep in range(3):
-        lab, msgs, _ = run_plan(data, table, 2, 1, iters, norm=0, dump_level=0)
+        lab, msgs, _ = run_plan(data, table, 2, 1, iters, norm=0, dump_level=0, compact=compact)
         assert int((msgs != ref_msgs).sum()) == 0, f"rep {rep}"
         np.testing.assert_array_equal(lab, ref_lab)
 
@@ -230,3 +237,40 @@ def test_host_entry_reuses_state_correctly(oracle_mod):
     np.testing.assert_array_equal(G.hbp_host(c, tc, 2, 2, 5), oracle_mod.hbp(c, tc, 2, 2, 5))
     assert G.lib().ginf_hbp_host_release() == 0
     np.testing.assert_array_equal(G.hbp_host(c, tc, 2, 2, 5), oracle_mod.hbp(c, tc, 2, 2, 5))
+
+
+# ---------------------------------------------------------------- 16-bit data costs
+
+def _pyramid_ref(oracle_mod, data, levels):
+    out = [data]
+    for _ in range(1, levels):
+        out.append(oracle_mod.pyramid(out[-1]))
+    return out
+
+
+@pytest.mark.parametrize("case", ["fits", "fraction", "coarse_overflow", "negative", "int_overflow"])
+def test_compact_data_fallback_exact(oracle_mod, case):
+    """16-bit storage is used exactly when every level's costs are integers in
+    [0, 65535]; otherwise the plan falls back to T storage.  Either way the
+    pyramid (dump_data), messages and labels equal the oracle's."""
+    rng = np.random.default_rng(17)
+    L, H, W, levels, iters = 16, 37, 52, 3, 6
+    dtype = np.int32 if case == "int_overflow" else np.float32
+    data = rng.integers(0, 40, size=(L, H, W)).astype(dtype)
+    if case == "fraction":
+        data[3, 10, 10] += 0.5
+    elif case == "coarse_overflow":  # fits at level 0, sums exceed 65535 at level 1
+        data[:, :4, :4] = 30000
+    elif case == "negative":
+        data[1, 5, 7] = -1
+    elif case == "int_overflow":
+        data[2, 20, 30] = 70000
+    table = oracle_mod.table_trunc_linear(L, 4, 12, dtype)
+    say(f"compact data case {case}")
+    ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, 2, levels, iters, norm=0, return_state=True)
+    lab, msgs, plan = run_plan(data, table, 2, levels, iters, norm=0, dump_level=0, kernel=2)
+    assert plan.compact_data() == (case == "fits")
+    np.testing.assert_array_equal(lab, ref_lab)
+    np.testing.assert_array_equal(msgs, ref_msgs)
+    for t, ref in enumerate(_pyramid_ref(oracle_mod, data, levels)):
+        np.testing.assert_array_equal(plan.dump_data(t), ref, err_msg=f"level {t}")
