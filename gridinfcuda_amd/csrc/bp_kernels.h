@@ -58,7 +58,8 @@ struct UpsampleArgs {
     int C, G;
 };
 
-// data16: the level's data costs are stored as 16-bit integers (quad kernel only).
+// data16: launch the quad instance that reads a.fit_flag and takes the data
+// costs in 16-bit storage when it is 0 (quad kernel only).
 hipError_t launch_sweep(int dtype, const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
                         hipStream_t st, bool data16 = false);
 // Per-family launchers (sweep_chunked_*.hip, sweep_quad.hip, sweep_general.hip).
@@ -76,11 +77,15 @@ hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, cons
                            void *dst0, void *dst1, hipStream_t st);
 // data16: store the level data as 16-bit integers; a value without an exact
 // 16-bit form sets *flag (which the caller clears beforehand) and is stored as 0.
+// only_if: when non-null the kernel does its work only if *only_if != 0 (the
+// redo in T after a 16-bit attempt that did not fit).
 hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
                                const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
-                               void *d11, hipStream_t st, bool data16 = false, int *flag = nullptr);
+                               void *d11, hipStream_t st, bool data16 = false, int *flag = nullptr,
+                               const int *only_if = nullptr);
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
-                          const LevelGeom &cg, int C, int G, hipStream_t st, bool data16 = false, int *flag = nullptr);
+                          const LevelGeom &cg, int C, int G, hipStream_t st, bool data16 = false, int *flag = nullptr,
+                          const int *only_if = nullptr);
 hipError_t launch_upsample(int dtype, const UpsampleArgs &u, hipStream_t st);
 // Batched persistent half-sweep (sweep_batch.hip): args[i] describes instance
 // i for the moving colour; items[k] = (instance, first slot, slots).

@@ -71,7 +71,8 @@ __device__ __forceinline__ void report_bad(bool bad, int *flag) {
 template <typename T, typename S, int TW>
 __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src, int W, int H, int L, StateLayout ly,
                                                       LevelGeom g0, S *d00, S *d01, int coarse, LevelGeom g1, S *d10,
-                                                      S *d11, int *flag) {
+                                                      S *d11, int *flag, const int *only_if) {
+    if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
     bool bad = false;
     extern __shared__ unsigned char smem_raw[];
     constexpr int TP = TW + 1;  // padded row: label-strided LDS reads hit distinct banks
@@ -173,7 +174,8 @@ __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src,
 // all four sit in slot column X of their rows.
 template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_pyramid(const S *f0, const S *f1, LevelGeom fg, S *c0, S *c1,
-                                                 LevelGeom cg, int C, int G, int *flag) {
+                                                 LevelGeom cg, int C, int G, int *flag, const int *only_if) {
+    if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int node = gid / G, q = gid - (gid / G) * G;
     const int X = node % cg.w, Y = node / cg.w;
@@ -299,7 +301,7 @@ hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, cons
 
 hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
                                const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
-                               void *d11, hipStream_t st, bool data16, int *flag) {
+                               void *d11, hipStream_t st, bool data16, int *flag, const int *only_if) {
     const int LP = ly.C * ly.G;
     const int tw = LP > 64 ? 32 : 64;
     const size_t lds = (size_t)2 * LP * (tw + 1) * 4;
@@ -307,7 +309,7 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
     const dim3 grid((unsigned)((W + tw - 1) / tw), (unsigned)((H + 1) / 2)), block(256);
 #define GINF_RP(TT, SS, TWW)                                                                                     \
     hipLaunchKernelGGL((k_relayout_pyr<TT, SS, TWW>), grid, block, lds, st, (const TT *)src, W, H, L, ly, g0,      \
-                       (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag)
+                       (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, only_if)
     if (data16) {
         if (dtype == 0) { if (tw == 32) GINF_RP(float, uint16_t, 32); else GINF_RP(float, uint16_t, 64); }
         else { if (tw == 32) GINF_RP(int, uint16_t, 32); else GINF_RP(int, uint16_t, 64); }
@@ -320,12 +322,13 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
 }
 
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
-                          const LevelGeom &cg, int C, int G, hipStream_t st, bool data16, int *flag) {
+                          const LevelGeom &cg, int C, int G, hipStream_t st, bool data16, int *flag,
+                          const int *only_if) {
     const int64_t threads = (int64_t)cg.w * cg.h * G;
     const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
 #define GINF_PY(TT, SS)                                                                                            \
     hipLaunchKernelGGL((k_pyramid<TT, SS>), grid, block, 0, st, (const SS *)f0, (const SS *)f1, fg, (SS *)c0,      \
-                       (SS *)c1, cg, C, G, flag)
+                       (SS *)c1, cg, C, G, flag, only_if)
     if (data16) { if (dtype == 0) GINF_PY(float, uint16_t); else GINF_PY(int, uint16_t); }
     else { if (dtype == 0) GINF_PY(float, float); else GINF_PY(int, int); }
 #undef GINF_PY

@@ -60,7 +60,8 @@ struct SweepArgs {
     uint32_t data_bytes;     // bytes of the data-cost array of the moving colour
     uint32_t in_span;        // bytes addressable from in_base
     int lmax;                // label capacity of the quad kernel instance (batched launches)
-    uint32_t data_plane_bytes;  // bytes between data-cost planes (half of plane_bytes for 16-bit costs)
+    uint32_t data_plane_bytes;  // bytes between data-cost planes in 16-bit storage (half of plane_bytes)
+    const int *fit_flag;     // 16-bit data path: 0 = the level data is stored in 16 bits, else in T
 };
 
 }  // namespace ginf

@@ -112,13 +112,12 @@ struct ginf_bp_plan_s {
     float s_f = 0, tau_f = 0;
     int s_i = 0, tau_i = 0;
     int device = 0;
-    // 16-bit data costs (quad kernel): on by default; a run uses them when the
-    // relayout and pyramid find every cost of every level to be an integer in
-    // [0, 65535] (flag_dev stays 0), else it redoes the level data in T.
+    // 16-bit data costs (quad kernel): on by default; a run keeps them when
+    // the relayout and pyramid find every cost of every level to be an integer
+    // in [0, 65535] (flag_dev stays 0), else it redoes the level data in T.
     bool compact = true;
     bool compact_last = false;
     int *flag_dev = nullptr;
-    int *flag_host = nullptr;  // pinned
     // sweep timing (profiling hook)
     bool timing = false;
     struct Timed { hipEvent_t beg, end; double updates; };
@@ -193,6 +192,16 @@ int64_t interior_of_color(int w, int h, int color) {
     return n;
 }
 
+// Whether the last run kept its data costs in 16-bit storage (waits for the device).
+hipError_t int16_used(ginf_bp_plan p, int *used) {
+    *used = 0;
+    if (!p->compact_last || !p->flag_dev) return hipSuccess;
+    int f = 1;
+    hipError_t e = hipMemcpy(&f, p->flag_dev, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) *used = f == 0 ? 1 : 0;
+    return e;
+}
+
 hipEvent_t take_event(ginf_bp_plan p) {
     if (!p->spare.empty()) { hipEvent_t e = p->spare.back(); p->spare.pop_back(); return e; }
     hipEvent_t e = nullptr;
@@ -213,30 +222,30 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
     const int dt = d.dtype, C = ly.C, G = ly.G, T = p->levels;
     const int W = d.width, H = d.height, L = d.num_labels;
     // Level 0 (and level 1) costs in one pass over the caller's array, then
-    // the coarser levels.  With 16-bit storage the host reads the fit flag
-    // once (one stream synchronisation per run) and falls back to T storage
-    // when a cost has no exact 16-bit form; both forms give the same values.
+    // the coarser levels.  With 16-bit storage every value is checked on the
+    // device; if any has no exact 16-bit form, conditional launches redo the
+    // level data in T and the sweeps (which read the same flag) take it from
+    // there.  Both forms hold the same values, and no host synchronisation is
+    // needed.
     const bool two = T > 1;
-    auto prepare = [&](bool d16) -> int {
-        int *flag = d16 ? p->flag_dev : nullptr;
+    auto prepare = [&](bool d16, int *flag, const int *only_if) -> int {
         GINF_HIP_TRY(launch_relayout_pyr(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0], p->lv[0].data[1],
                                          two, p->geom[two ? 1 : 0], p->lv[two ? 1 : 0].data[0],
-                                         p->lv[two ? 1 : 0].data[1], st, d16, flag));
+                                         p->lv[two ? 1 : 0].data[1], st, d16, flag, only_if));
         for (int t = 2; t <= start; ++t)
             GINF_HIP_TRY(launch_pyramid(dt, p->lv[t - 1].data[0], p->lv[t - 1].data[1], p->geom[t - 1],
-                                        p->lv[t].data[0], p->lv[t].data[1], p->geom[t], C, G, st, d16, flag));
+                                        p->lv[t].data[0], p->lv[t].data[1], p->geom[t], C, G, st, d16, flag,
+                                        only_if));
         return GINF_OK;
     };
-    bool d16 = p->compact && p->shape.quad && p->flag_dev && p->flag_host;
+    const bool d16 = p->compact && p->shape.quad && p->flag_dev;
     if (d16) {
         GINF_HIP_TRY(launch_fill_u32((uint32_t *)p->flag_dev, 1, 0u, st));
-        if (int rc = prepare(true)) return rc;
-        GINF_HIP_TRY(hipMemcpyAsync(p->flag_host, p->flag_dev, sizeof(int), hipMemcpyDeviceToHost, st));
-        GINF_HIP_TRY(hipStreamSynchronize(st));
-        if (*p->flag_host) d16 = false;
+        if (int rc = prepare(true, p->flag_dev, nullptr)) return rc;
+        if (int rc = prepare(false, nullptr, p->flag_dev)) return rc;
+    } else if (int rc = prepare(false, nullptr, nullptr)) {
+        return rc;
     }
-    if (!d16)
-        if (int rc = prepare(false)) return rc;
     p->compact_last = d16;
     for (int t = start; t >= 0; --t) {
         const LevelGeom &g = p->geom[t];
@@ -293,9 +302,10 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
             a.table = p->table_dev;
             // byte view for the quad kernel
             a.arr_bytes = (uint32_t)p->arr_bytes[t];
-            a.data_bytes = (uint32_t)(d16 ? p->arr_bytes[t] / 2 : p->arr_bytes[t]);
+            a.data_bytes = (uint32_t)p->arr_bytes[t];
             a.plane_bytes = (uint32_t)(g.nplane * G * 4);
-            a.data_plane_bytes = d16 ? a.plane_bytes / 2 : a.plane_bytes;
+            a.data_plane_bytes = a.plane_bytes / 2;
+            a.fit_flag = d16 ? p->flag_dev : nullptr;
             a.out_base = p->lv[t].msgs[color][0];
             if (src_level == t) {
                 a.in_base = p->lv[t].msgs[1 - color][0];
@@ -437,11 +447,9 @@ int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, gin
     }
     if (e == hipSuccess && p->shape.quad) {
         e = hipMalloc((void **)&p->flag_dev, sizeof(int));
-        if (e == hipSuccess) e = hipHostMalloc((void **)&p->flag_host, sizeof(int), hipHostMallocDefault);
     }
     if (e != hipSuccess) {
         if (p->flag_dev) hipFree(p->flag_dev);
-        if (p->flag_host) hipHostFree(p->flag_host);
         hipFree(p->pool);
         delete p;
         return set_error(GINF_ERR_HIP, "plan set-up failed: %s", hipGetErrorString(e));
@@ -456,7 +464,6 @@ int ginf_bp_plan_destroy(ginf_bp_plan plan) {
     for (auto e : plan->spare) hipEventDestroy(e);
     if (plan->pool) hipFree(plan->pool);
     if (plan->flag_dev) hipFree(plan->flag_dev);
-    if (plan->flag_host) hipHostFree(plan->flag_host);
     delete plan;
     return GINF_OK;
 }
@@ -475,7 +482,7 @@ int ginf_bp_plan_set_compact_data(ginf_bp_plan plan, int enable) {
 
 int ginf_bp_plan_compact_data(ginf_bp_plan plan, int *last_run_compact) {
     if (!plan || !last_run_compact) return set_error(GINF_ERR_INVALID, "NULL argument");
-    *last_run_compact = plan->compact_last ? 1 : 0;
+    int16_used(plan, last_run_compact);
     return GINF_OK;
 }
 
@@ -604,11 +611,13 @@ int ginf_bp_plan_dump_data(ginf_bp_plan p, int level, void *data_host) {
     const int C = ly.C, G = ly.G;
     const size_t n = (size_t)C * g.nplane * G;
     std::vector<uint32_t> buf(n);
-    std::vector<uint16_t> buf16(p->compact_last ? n : 0);
-    uint32_t *dst = (uint32_t *)data_host;
     GINF_HIP_TRY(hipDeviceSynchronize());
+    int used16 = 0;
+    GINF_HIP_TRY(int16_used(p, &used16));
+    std::vector<uint16_t> buf16(used16 ? n : 0);
+    uint32_t *dst = (uint32_t *)data_host;
     for (int c = 0; c < 2; ++c) {
-        if (p->compact_last) {  // 16-bit storage of the last run: widen to T
+        if (used16) {  // 16-bit storage of the last run: widen to T
             GINF_HIP_TRY(hipMemcpy(buf16.data(), p->lv[level].data[c], n * 2, hipMemcpyDeviceToHost));
             for (size_t k = 0; k < n; ++k) {
                 if (p->desc.dtype == GINF_F32) {

@@ -35,9 +35,11 @@ __device__ __forceinline__ int quad_bcast(int v) {
 // the reference's sequential scans and left-fold normaliser, so every
 // operation -- FP32 mean mode included -- rounds exactly as src/bp.cc:26-54.
 //   LMAX  register capacity in labels; FULL: L == LMAX (no label guards)
-//   D16   data costs stored as 16-bit unsigned integers (exact: the plan only
-//         uses this form when every cost of the level is an integer in
-//         [0, 65535]); a lane loads its 4 labels' costs as 8 bytes
+//   D16   instance for plans with 16-bit data storage: *a.fit_flag == 0 means
+//         the costs were stored as 16-bit unsigned integers (exact: every cost
+//         of every level is an integer in [0, 65535]) and a lane loads its 4
+//         labels' costs as 8 bytes; otherwise they were stored in T.  The flag
+//         is uniform, so the branch never diverges.
 // Layout: label quads, element (l, slot) at ((l/4) * nplane + slot) * 4 + l%4,
 // so one lane moves 4 labels (16 B) per access and 16 consecutive nodes of a
 // wave cover 256 contiguous bytes of every array.  Offsets are 32-bit and
@@ -45,8 +47,8 @@ __device__ __forceinline__ int quad_bcast(int v) {
 // soffset, the compiler reused SGPRs that a v_cmp in the labelling code had
 // just written, and the memory instruction then intermittently read the wrong
 // offset (nondeterministic message corruption, measured on gfx950).
-template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool D16 = false>
-__device__ __forceinline__ void quad_node(const SweepArgs &a, const int slot, const int d) {
+template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool use16>
+__device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slot, const int d) {
     typedef T v4 __attribute__((ext_vector_type(4)));
     constexpr int LQ = LMAX / 4;
     if (slot >= a.nslots) return;
@@ -61,8 +63,8 @@ __device__ __forceinline__ void quad_node(const SweepArgs &a, const int slot, co
     const __amdgpu_buffer_rsrc_t irs = __builtin_amdgcn_make_buffer_rsrc((void *)a.in_base, 0, a.in_span, 0x00020000);
     const __amdgpu_buffer_rsrc_t ors = __builtin_amdgcn_make_buffer_rsrc(a.out_base, 0, 4u * a.arr_bytes, 0x00020000);
     const uint32_t self_off = (uint32_t)slot * 16u;
-    const uint32_t data_off = D16 ? (uint32_t)slot * 8u : self_off;
-    const uint32_t data_plane = D16 ? a.data_plane_bytes : a.plane_bytes;
+    const uint32_t data_off = use16 ? (uint32_t)slot * 8u : self_off;
+    const uint32_t data_plane = use16 ? a.data_plane_bytes : a.plane_bytes;
     uint32_t in_off = 0;
     if (!ZEROIN) {
         int nx = x + kDX[d], ny = y + kDY[d];
@@ -88,7 +90,7 @@ __device__ __forceinline__ void quad_node(const SweepArgs &a, const int slot, co
             if (!ZEROIN)
                 m4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(irs, in_off + (uint32_t)j * a.in_plane_bytes, 0, GINF_QUAD_LOAD_POLICY));
             v4 d4;
-            if (D16) {
+            if constexpr (use16) {
                 const auto h = __builtin_amdgcn_raw_buffer_load_b64(drs, data_off + (uint32_t)j * data_plane, 0, GINF_QUAD_LOAD_POLICY);
                 d4[0] = T(h[0] & 0xFFFFu);
                 d4[1] = T(h[0] >> 16);
@@ -163,6 +165,20 @@ __device__ __forceinline__ void quad_node(const SweepArgs &a, const int slot, co
         }
         if (d == 0) a.labels[(int64_t)y * a.label_w + x] = arg;
     }
+}
+
+// The two storage forms are separate straight-line bodies; the uniform flag
+// picks one per wave (a branch inside the unrolled label loop would stop the
+// compiler from issuing the loads ahead).
+template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool D16 = false>
+__device__ __forceinline__ void quad_node(const SweepArgs &a, const int slot, const int d) {
+    if constexpr (D16) {
+        if (__builtin_amdgcn_readfirstlane(*a.fit_flag) == 0) {
+            quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, true>(a, slot, d);
+            return;
+        }
+    }
+    quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, false>(a, slot, d);
 }
 
 }  // namespace

@@ -75,21 +75,19 @@ int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, gin
 /* Run HBP on device-resident costs data_dev (L x H x W, x fastest, the layout
  * of Grid<T>::getDataCosts()) and write interior labels of labels_dev
  * (H x W int32, x fastest); border labels are left untouched, as the
- * reference leaves them.  Asynchronous on `stream`, except that with 16-bit
- * data costs enabled (ginf_bp_plan_set_compact_data, the default for the
- * quad kernel) the host waits once per run for the cost check. */
+ * reference leaves them.  Asynchronous on `stream`. */
 int ginf_bp_plan_run(ginf_bp_plan plan, const void *data_dev, int32_t *labels_dev, void *stream);
 
 int ginf_bp_plan_destroy(ginf_bp_plan plan);
 
 /* 16-bit data costs (no reference counterpart).  With the quad kernel a
  * plan stores the pyramid's data costs as 16-bit integers whenever every
- * cost of every level is an integer in [0, 65535] (checked on the device
- * each run; the host reads the result once per run, which synchronises with
- * the run's stream), and as T otherwise; both give identical results, the
- * 16-bit form moves 34 instead of 36 bytes per node-label update.  Enabled by
- * default; enable = 0 keeps T storage and a fully asynchronous run.
- * ginf_bp_plan_compact_data reports whether the last run used 16-bit costs. */
+ * cost of every level is an integer in [0, 65535], and as T otherwise; the
+ * check and the choice happen on the device during the run (the sweeps read
+ * the outcome), both forms give identical results, and the 16-bit form moves
+ * 34 instead of 36 bytes per node-label update.  Enabled by default; enable
+ * = 0 always stores T.  ginf_bp_plan_compact_data reports whether the last
+ * run kept 16-bit costs (it waits for the device). */
 int ginf_bp_plan_set_compact_data(ginf_bp_plan plan, int enable);
 int ginf_bp_plan_compact_data(ginf_bp_plan plan, int *last_run_compact);
 
