@@ -30,8 +30,10 @@ def _declare():
     L.ginf_batch_destroy.argtypes = [VP]
     L.ginf_batch_set_seeded.argtypes = [VP, I]
     L.ginf_batch_seed.argtypes = [VP, I, VP, I, I, I, ctypes.c_float, VP]
+    L.ginf_batch_set_compact_data.argtypes = [VP, I]
     for n in ("ginf_batch_create", "ginf_batch_run", "ginf_batch_set_tiles", "ginf_batch_updates_per_run",
-              "ginf_batch_dump_msgs", "ginf_batch_destroy", "ginf_batch_set_seeded", "ginf_batch_seed"):
+              "ginf_batch_dump_msgs", "ginf_batch_destroy", "ginf_batch_set_seeded", "ginf_batch_seed",
+              "ginf_batch_set_compact_data"):
         getattr(L, n).restype = I
     _declared = True
 
@@ -60,6 +62,9 @@ class Batch:
         dp = (ctypes.c_void_p * self.n)(*data_ptrs)
         lp = (ctypes.c_void_p * self.n)(*label_ptrs)
         G.check(G.lib().ginf_batch_run(self._h, dp, lp, ctypes.c_void_p(stream)))
+
+    def set_compact_data(self, enable: bool) -> None:
+        G.check(G.lib().ginf_batch_set_compact_data(self._h, 1 if enable else 0))
 
     def set_seeded(self, seeded: bool) -> None:
         G.check(G.lib().ginf_batch_set_seeded(self._h, 1 if seeded else 0))

@@ -97,6 +97,10 @@ struct BatchItem {
 };
 hipError_t launch_sweep_batch(int dtype, const SweepArgs *args_dev, const BatchItem *items_dev, int nitems, int grid,
                               bool zeroin, bool labels, hipStream_t st);
+// The same with each instance's data costs in 16-bit storage when its
+// SweepArgs::fit_flag reads 0 (sweep_batch16.hip).
+hipError_t launch_sweep_batch16(int dtype, const SweepArgs *args_dev, const BatchItem *items_dev, int nitems, int grid,
+                                bool zeroin, bool labels, hipStream_t st);
 
 // Seed a level's outboxes from an NHWC prediction (conv_net.hip).
 hipError_t launch_inject(const float *pred, void *const msgs[2][4], const LevelGeom &g, int C, int G, bool interleave,

@@ -732,6 +732,7 @@ struct ginf_batch_s {
     int blocks_per_cu = 4;
     int num_cu = 0;
     bool seeded = false;  // first half-sweep reads the outboxes stored by ginf_batch_seed
+    bool compact = true;  // 16-bit data costs per instance where they fit (each plan's fit flag)
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -797,10 +798,20 @@ int batch_capture(ginf_batch b, const void *const *data_dev, int32_t *const *lab
     for (int i = 0; i < b->n && rc == GINF_OK; ++i) {
         const ginf_bp_plan p = b->plans[i];
         const StateLayout ly = state_layout(p->shape, p->desc.num_labels);
-        hipError_t e = launch_relayout_pyr(b->dtype, data_dev[i], p->desc.width, p->desc.height, p->desc.num_labels, ly,
-                                           p->geom[0], p->lv[0].data[0], p->lv[0].data[1], false, p->geom[0],
-                                           p->lv[0].data[0], p->lv[0].data[1], st);
-        if (e != hipSuccess) return fail(e);
+        auto relayout = [&](bool d16, int *flag, const int *only_if) {
+            return launch_relayout_pyr(b->dtype, data_dev[i], p->desc.width, p->desc.height, p->desc.num_labels, ly,
+                                       p->geom[0], p->lv[0].data[0], p->lv[0].data[1], false, p->geom[0],
+                                       p->lv[0].data[0], p->lv[0].data[1], st, d16, flag, only_if);
+        };
+        hipError_t e;
+        if (b->compact) {
+            // 16-bit costs where the instance's costs fit, else redone in T (decided on the device)
+            if ((e = launch_fill_u32(p->flag_dev, 1, 0u, st)) != hipSuccess) return fail(e);
+            if ((e = relayout(true, p->flag_dev, nullptr)) != hipSuccess) return fail(e);
+            if ((e = relayout(false, nullptr, p->flag_dev)) != hipSuccess) return fail(e);
+        } else if ((e = relayout(false, nullptr, nullptr)) != hipSuccess) {
+            return fail(e);
+        }
         if (N <= 1) {
             if (N == 1 && !b->seeded) {
                 const int64_t per = (int64_t)ly.C * p->geom[0].nplane * ly.G;
@@ -818,8 +829,8 @@ int batch_capture(ginf_batch b, const void *const *data_dev, int32_t *const *lab
         const int color = (it + 1) & 1;
         const bool zeroin = N >= 2 && it == 0 && !b->seeded;
         const bool labels = it >= N - 2;
-        hipError_t e = launch_sweep_batch(b->dtype, b->args_dev + (size_t)color * b->n, b->items_dev, b->nitems, grid,
-                                          zeroin, labels, st);
+        hipError_t e = (b->compact ? launch_sweep_batch16 : launch_sweep_batch)(
+            b->dtype, b->args_dev + (size_t)color * b->n, b->items_dev, b->nitems, grid, zeroin, labels, st);
         if (e != hipSuccess) return fail(e);
     }
     hipGraph_t g = nullptr;
@@ -883,7 +894,9 @@ int ginf_batch_create(int n, const ginf_bp_desc *descs, const void *const *table
             a.s_f = p->s_f; a.tau_f = p->tau_f; a.s_i = p->s_i; a.tau_i = p->tau_i;
             a.table = p->table_dev;
             a.arr_bytes = a.data_bytes = a.in_arr_bytes = (uint32_t)p->arr_bytes[0];
-            a.plane_bytes = a.in_plane_bytes = a.data_plane_bytes = (uint32_t)(g.nplane * ly.G * 4);
+            a.plane_bytes = a.in_plane_bytes = (uint32_t)(g.nplane * ly.G * 4);
+            a.data_plane_bytes = a.plane_bytes / 2;
+            a.fit_flag = p->flag_dev;
             a.out_base = p->lv[0].msgs[c][0];
             a.in_base = p->lv[0].msgs[1 - c][0];
             a.in_span = (uint32_t)(4 * p->arr_bytes[0]);
@@ -929,6 +942,13 @@ int ginf_batch_run(ginf_batch b, const void *const *data_dev, int32_t *const *la
         if (rc) return rc;
     }
     GINF_HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return GINF_OK;
+}
+
+int ginf_batch_set_compact_data(ginf_batch b, int enable) {
+    if (!b) return set_error(GINF_ERR_INVALID, "NULL batch");
+    if (b->compact != (enable != 0)) batch_drop_graph(b);
+    b->compact = enable != 0;
     return GINF_OK;
 }
 

@@ -126,3 +126,27 @@ def test_seeded_batch(oracle_mod):
         want = oracle_mod.labels_from_belief(bel, np.zeros((H, W), np.int32))
         np.testing.assert_array_equal(ll[k].cpu().numpy()[1:-1, 1:-1], want[1:-1, 1:-1], err_msg=f"inst {k}")
         np.testing.assert_array_equal(bt.dump_msgs(k), m, err_msg=f"inst {k}")
+
+
+@pytest.mark.parametrize("compact", [True, False])
+def test_batch_data_storage_forms(oracle_mod, compact):
+    """Each instance keeps 16-bit data costs when they fit and falls back on
+    its own when they do not (here: one fractional and one out-of-range
+    instance among integer ones); with either setting every instance equals
+    the oracle."""
+    torch = torch_cuda()
+    say(f"batch data storage compact={compact}")
+    insts = small_instances(8, 123)
+    datas = [B.instance_costs(x) for x in insts]
+    datas[2] = datas[2] + np.float32(0.25)
+    datas[5] = datas[5].copy()
+    datas[5][0, 1, 1] = 70000.0
+    bt = B.make_batch(insts, iters=5)
+    bt.set_compact_data(compact)
+    inits = [np.zeros((x["H"], x["W"]), np.int32) for x in insts]
+    dd, ll = run_batch(torch, bt, datas, inits)
+    for k, x in enumerate(insts):
+        ref_lab, ref_msgs, _ = oracle_mod.hbp(datas[k], x["table"], x["model"], 1, 5, norm=0,
+                                              labels_init=inits[k], return_state=True)
+        np.testing.assert_array_equal(ll[k].cpu().numpy(), ref_lab, err_msg=f"instance {k}")
+        np.testing.assert_array_equal(bt.dump_msgs(k), ref_msgs, err_msg=f"instance {k}")
