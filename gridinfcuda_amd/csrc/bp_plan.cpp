@@ -52,6 +52,19 @@ namespace {
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+// Layout knobs.  GINF_PLANE_PAD: extra slots between label planes of a level
+// with at least 64K slots per plane; 256 slots (4 KiB with label quads) took
+// the 1920x1080x128 half-sweep from 215.5 to 213.5 us per launch (whole decode
+// +1.2 %), while 0, 192 and 512 did not.  GINF_ARRAY_STAGGER: extra bytes
+// between a level's state arrays (0 = packed; 4, 8 and 68 KiB staggers were
+// all slower, 220-221 us).
+#ifndef GINF_ARRAY_STAGGER
+#define GINF_ARRAY_STAGGER 0
+#endif
+#ifndef GINF_PLANE_PAD
+#define GINF_PLANE_PAD 256
+#endif
+
 // Default register tile per label count for the O(L) kernel: the fewest
 // lanes per node that keep C <= 16 labels per lane.
 bool pick_linear_shape(int L, int lanes, ginf::SweepShape &sh) {
@@ -406,12 +419,15 @@ int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, gin
         g.w = w; g.h = h;
         g.whp = (int)round_up((w + 1) / 2, 16);
         g.nslots = g.whp * h;
-        g.nplane = round_up(g.nslots, 64);
+        g.nplane = round_up(g.nslots, 64) + (g.nslots >= 65536 ? GINF_PLANE_PAD : 0);
         p->geom.push_back(g);
         const ginf::StateLayout ly = ginf::state_layout(sh, L);
         const size_t arr = (size_t)ly.C * g.nplane * ly.G * es;
-        for (int k = 0; k < 10; ++k) { offs.push_back(total); total += round_up(arr, 256); }
-        p->arr_bytes.push_back(round_up(arr, 256));
+        const size_t stride = (GINF_ARRAY_STAGGER > 0 && arr >= ((size_t)1 << 20))
+                                  ? round_up(arr, (size_t)1 << 20) + GINF_ARRAY_STAGGER
+                                  : round_up(arr, 256);
+        for (int k = 0; k < 10; ++k) { offs.push_back(total); total += stride; }
+        p->arr_bytes.push_back(stride);
     }
     total += round_up((size_t)L * L * es, 256);
     if (sh.quad) {
