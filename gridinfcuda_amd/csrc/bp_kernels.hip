@@ -36,6 +36,12 @@ __global__ __launch_bounds__(256) void k_relayout(const T *__restrict__ src, int
 // when `coarse` is set, the level-1 costs of coarse row Y: the children
 // (2X,2Y), (2X+1,2Y), (2X,2Y+1), (2X+1,2Y+1) added in that order from 0
 // (src/bp.cc:100-117).  Padding labels (L <= l < C*G) get cost 0.
+// Widest label count (C*G) whose relayout tile is 64 columns wide (else 32):
+// the tile holds 2 rows x C*G labels x (TW+1) values in LDS.
+#ifndef GINF_RELAYOUT_TW64_MAX_LP
+#define GINF_RELAYOUT_TW64_MAX_LP 64
+#endif
+
 // Storage of a data cost: T itself, or a 16-bit unsigned integer when S is
 // uint16_t.  to_store records in *bad when a value has no exact 16-bit form
 // (not an integer in [0, 65535], or -0), and the plan then redoes the level
@@ -172,29 +178,32 @@ __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src,
 // (2X,2Y), (2X+1,2Y), (2X,2Y+1), (2X+1,2Y+1) in that order, starting from 0.
 // Children (2X,2Y) and (2X+1,2Y+1) have colour 0, the other two colour 1, and
 // all four sit in slot column X of their rows.
+// One thread per (plane i, coarse node, lane q): four loads and one store,
+// so the small coarse levels are not a chain of C dependent plane steps.
 template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_pyramid(const S *f0, const S *f1, LevelGeom fg, S *c0, S *c1,
                                                  LevelGeom cg, int C, int G, int *flag, const int *only_if) {
     if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int node = gid / G, q = gid - (gid / G) * G;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per_plane = (int64_t)cg.w * cg.h * G;
+    const int i = (int)(gid / per_plane);
+    if (i >= C) return;
+    const int r = (int)(gid - (int64_t)i * per_plane);
+    const int node = r / G, q = r - node * G;
     const int X = node % cg.w, Y = node / cg.w;
-    if (Y >= cg.h) return;
     const bool ex = 2 * X + 1 < fg.w, ey = 2 * Y + 1 < fg.h;
     const int64_t s00 = (int64_t)(2 * Y) * fg.whp + X;
     const int64_t s01 = (int64_t)(2 * Y + 1) * fg.whp + X;
     S *dst = ((X + Y) & 1) ? c1 : c0;
     const int64_t cs = (int64_t)Y * cg.whp + (X >> 1);
     bool bad = false;
-    for (int i = 0; i < C; ++i) {
-        const int64_t fo = (int64_t)i * fg.nplane;
-        T acc = T(0);
-        acc = acc + from_store<S, T>(f0[(fo + s00) * G + q]);
-        if (ex) acc = acc + from_store<S, T>(f1[(fo + s00) * G + q]);
-        if (ey) acc = acc + from_store<S, T>(f1[(fo + s01) * G + q]);
-        if (ex && ey) acc = acc + from_store<S, T>(f0[(fo + s01) * G + q]);
-        dst[((int64_t)i * cg.nplane + cs) * G + q] = to_store<S>(acc, bad);
-    }
+    const int64_t fo = (int64_t)i * fg.nplane;
+    T acc = T(0);
+    acc = acc + from_store<S, T>(f0[(fo + s00) * G + q]);
+    if (ex) acc = acc + from_store<S, T>(f1[(fo + s00) * G + q]);
+    if (ey) acc = acc + from_store<S, T>(f1[(fo + s01) * G + q]);
+    if (ex && ey) acc = acc + from_store<S, T>(f0[(fo + s01) * G + q]);
+    dst[((int64_t)i * cg.nplane + cs) * G + q] = to_store<S>(acc, bad);
     report_bad(bad, flag);
 }
 
@@ -303,7 +312,7 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
                                const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
                                void *d11, hipStream_t st, bool data16, int *flag, const int *only_if) {
     const int LP = ly.C * ly.G;
-    const int tw = LP > 64 ? 32 : 64;
+    const int tw = (LP <= GINF_RELAYOUT_TW64_MAX_LP && 32 * ly.G <= 256) ? 64 : 32;
     const size_t lds = (size_t)2 * LP * (tw + 1) * 4;
     if (lds > 160 * 1024 || (tw / 2) * ly.G > 256) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((W + tw - 1) / tw), (unsigned)((H + 1) / 2)), block(256);
@@ -324,7 +333,7 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
                           const LevelGeom &cg, int C, int G, hipStream_t st, bool data16, int *flag,
                           const int *only_if) {
-    const int64_t threads = (int64_t)cg.w * cg.h * G;
+    const int64_t threads = (int64_t)cg.w * cg.h * G * C;
     const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
 #define GINF_PY(TT, SS)                                                                                            \
     hipLaunchKernelGGL((k_pyramid<TT, SS>), grid, block, 0, st, (const SS *)f0, (const SS *)f1, fg, (SS *)c0,      \
