@@ -284,6 +284,75 @@ __global__ __launch_bounds__(256) void k_energy_final(const double *partials, in
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
+
+// Relayout for the label-quad layout (l = 4i + q), without LDS: block
+// (column-pair run, row pair Y, quad i); thread k takes columns 2k, 2k+1 of
+// rows 2Y, 2Y+1 for labels 4i..4i+3 (eight 8-byte loads from rows that a wave
+// reads as 512-byte runs), writes the four level-0 nodes as one quad each
+// (slot k of row y in the node's colour) and, fused, the level-1 node (k, Y)
+// as the child sum in the reference order (2X,2Y), (2X+1,2Y), (2X,2Y+1),
+// (2X+1,2Y+1).  Labels past L are stored as 0, as the tiled kernel does.
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src, int W, int H, int L, LevelGeom g0,
+                                                       S *d00, S *d01, int coarse, LevelGeom g1, S *d10, S *d11,
+                                                       int *flag, const int *only_if) {
+    if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // column pair
+    const int Y = blockIdx.y, i = blockIdx.z;
+    const int x0 = 2 * k, y0 = 2 * Y;
+    if (x0 >= W) return;
+    const bool ex = x0 + 1 < W, ey = y0 + 1 < H;
+    T v[4][2][2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int l = 4 * i + q;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            v[q][r][0] = T(0);
+            v[q][r][1] = T(0);
+            if (l < L && (r == 0 || ey)) {
+                const T *row = src + ((int64_t)l * H + y0 + r) * W;
+                v[q][r][0] = row[x0];
+                if (ex) v[q][r][1] = row[x0 + 1];
+            }
+        }
+    }
+    bool bad = false;
+    typedef S s4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        if (r == 1 && !ey) break;
+        const int y = y0 + r;
+        const int64_t slot = (int64_t)y * g0.whp + k;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (c == 1 && !ex) break;
+            const int x = x0 + c;
+            S *dst = ((x + y) & 1) ? d01 : d00;
+            s4 o;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[q] = to_store<S>(v[q][r][c], bad);
+            *(s4 *)(dst + ((int64_t)i * g0.nplane + slot) * 4) = o;
+        }
+    }
+    if (coarse && k < g1.w) {
+        const int64_t slot = (int64_t)Y * g1.whp + (k >> 1);
+        S *dst = ((k + Y) & 1) ? d11 : d10;
+        s4 o;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            T acc = T(0);
+            acc = acc + v[q][0][0];
+            if (ex) acc = acc + v[q][0][1];
+            if (ey) acc = acc + v[q][1][0];
+            if (ex && ey) acc = acc + v[q][1][1];
+            o[q] = to_store<S>(acc, bad);
+        }
+        *(s4 *)(dst + ((int64_t)i * g1.nplane + slot) * 4) = o;
+    }
+    report_bad(bad, flag);
+}
+
 }  // namespace
 
 hipError_t launch_sweep(int dtype, const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
@@ -311,6 +380,17 @@ hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, cons
 hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
                                const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
                                void *d11, hipStream_t st, bool data16, int *flag, const int *only_if) {
+    if (ly.interleave && ly.G == 4) {  // label quads: register-only relayout
+        const dim3 grid((unsigned)((W + 511) / 512), (unsigned)((H + 1) / 2), (unsigned)ly.C), block(256);
+        if (ly.C > 65535 || (H + 1) / 2 > 65535) return hipErrorInvalidValue;
+#define GINF_RQ(TT, SS)                                                                                           \
+        hipLaunchKernelGGL((k_relayout_quad<TT, SS>), grid, block, 0, st, (const TT *)src, W, H, L, g0, (SS *)d00, \
+                           (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, only_if)
+        if (data16) { if (dtype == 0) GINF_RQ(float, uint16_t); else GINF_RQ(int, uint16_t); }
+        else { if (dtype == 0) GINF_RQ(float, float); else GINF_RQ(int, int); }
+#undef GINF_RQ
+        return hipGetLastError();
+    }
     const int LP = ly.C * ly.G;
     const int tw = (LP <= GINF_RELAYOUT_TW64_MAX_LP && 32 * ly.G <= 256) ? 64 : 32;
     const size_t lds = (size_t)2 * LP * (tw + 1) * 4;
