@@ -19,7 +19,7 @@ struct SweepShape {
 };
 
 // How a node's labels map onto the (plane i, lane q) grid of the state arrays
-// (element (i * nplane + slot) * G + q): label l = q * C + i, or with
+// (element elem_index(slot, i, q, C, G) of bp_layout.h): label l = q * C + i, or with
 // `interleave` l = i * G + q (the quad kernel's 4-label groups).
 struct StateLayout {
     int C;

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(256) void k_relayout(const T *__restrict__ src, int
         const int qq = ly.interleave ? l % ly.G : l / ly.C;
         const int i = ly.interleave ? l / ly.G : l - qq * ly.C;
         const T v = (l < L) ? src[((int64_t)l * H + y) * W + x] : T(0);
-        dst[((int64_t)i * g.nplane + slot) * ly.G + qq] = v;
+        dst[elem_index(slot, i, qq, ly.C, ly.G)] = v;
     }
 }
 
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src,
                 S *dst = c ? d01 : d00;
                 for (int i = sub; i < C; i += ni) {
                     const int l = ly.interleave ? (i << gs) + q : q * C + i;
-                    dst[((int64_t)i * g0.nplane + slot) * G + q] = to_store<S>(tile[(rr * LP + l) * TP + xl], bad);
+                    dst[elem_index(slot, i, q, C, G)] = to_store<S>(tile[(rr * LP + l) * TP + xl], bad);
                 }
             }
         }
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src,
                 if (ex) acc = acc + t0[2 * Xl + 1];
                 if (ey) acc = acc + t1[2 * Xl];
                 if (ex && ey) acc = acc + t1[2 * Xl + 1];
-                dst[((int64_t)i * g1.nplane + slot) * G + q] = to_store<S>(acc, bad);
+                dst[elem_index(slot, i, q, C, G)] = to_store<S>(acc, bad);
             }
         }
     }
@@ -197,13 +197,13 @@ __global__ __launch_bounds__(256) void k_pyramid(const S *f0, const S *f1, Level
     S *dst = ((X + Y) & 1) ? c1 : c0;
     const int64_t cs = (int64_t)Y * cg.whp + (X >> 1);
     bool bad = false;
-    const int64_t fo = (int64_t)i * fg.nplane;
+    const int64_t e00 = elem_index(s00, i, q, C, G), e01 = elem_index(s01, i, q, C, G);
     T acc = T(0);
-    acc = acc + from_store<S, T>(f0[(fo + s00) * G + q]);
-    if (ex) acc = acc + from_store<S, T>(f1[(fo + s00) * G + q]);
-    if (ey) acc = acc + from_store<S, T>(f1[(fo + s01) * G + q]);
-    if (ex && ey) acc = acc + from_store<S, T>(f0[(fo + s01) * G + q]);
-    dst[((int64_t)i * cg.nplane + cs) * G + q] = to_store<S>(acc, bad);
+    acc = acc + from_store<S, T>(f0[e00]);
+    if (ex) acc = acc + from_store<S, T>(f1[e00]);
+    if (ey) acc = acc + from_store<S, T>(f1[e01]);
+    if (ex && ey) acc = acc + from_store<S, T>(f0[e01]);
+    dst[elem_index(cs, i, q, C, G)] = to_store<S>(acc, bad);
     report_bad(bad, flag);
 }
 
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_upsample(UpsampleArgs u) {
         const T *src = (const T *)u.coarse[pc][d];
         T *dst = (T *)u.fine[fc][d];
         for (int i = 0; i < u.C; ++i)
-            dst[((int64_t)i * u.fg.nplane + fs) * G + q] = src[((int64_t)i * u.cg.nplane + ps) * G + q];
+            dst[elem_index(fs, i, q, u.C, G)] = src[elem_index(ps, i, q, u.C, G)];
     }
 }
 
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src
                                                        int *flag, const int *only_if) {
     if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
     const int k = blockIdx.x * blockDim.x + threadIdx.x;  // column pair
-    const int Y = blockIdx.y, i = blockIdx.z;
+    const int Y = blockIdx.y, i = blockIdx.z, C = gridDim.z;  // one z-block per plane
     const int x0 = 2 * k, y0 = 2 * Y;
     if (x0 >= W) return;
     const bool ex = x0 + 1 < W, ey = y0 + 1 < H;
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src
             s4 o;
 #pragma unroll
             for (int q = 0; q < 4; ++q) o[q] = to_store<S>(v[q][r][c], bad);
-            *(s4 *)(dst + ((int64_t)i * g0.nplane + slot) * 4) = o;
+            *(s4 *)(dst + elem_index(slot, i, 0, C, 4)) = o;
         }
     }
     if (coarse && k < g1.w) {
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src
             if (ex && ey) acc = acc + v[q][1][1];
             o[q] = to_store<S>(acc, bad);
         }
-        *(s4 *)(dst + ((int64_t)i * g1.nplane + slot) * 4) = o;
+        *(s4 *)(dst + elem_index(slot, i, 0, C, 4)) = o;
     }
     report_bad(bad, flag);
 }

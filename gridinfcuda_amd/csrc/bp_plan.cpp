@@ -52,17 +52,11 @@ namespace {
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
-// Layout knobs.  GINF_PLANE_PAD: extra slots between label planes of a level
-// with at least 64K slots per plane; 256 slots (4 KiB with label quads) took
-// the 1920x1080x128 half-sweep from 215.5 to 213.5 us per launch (whole decode
-// +1.2 %), while 0, 192 and 512 did not.  GINF_ARRAY_STAGGER: extra bytes
-// between a level's state arrays (0 = packed; 4, 8 and 68 KiB staggers were
-// all slower, 220-221 us).
+// Layout knob.  GINF_ARRAY_STAGGER: extra bytes between a level's state
+// arrays (0 = packed; with plane-major arrays, 4, 8 and 68 KiB staggers were
+// all slower on the 1920x1080x128 half-sweep).
 #ifndef GINF_ARRAY_STAGGER
 #define GINF_ARRAY_STAGGER 0
-#endif
-#ifndef GINF_PLANE_PAD
-#define GINF_PLANE_PAD 256
 #endif
 
 // Default register tile per label count for the O(L) kernel: the fewest
@@ -302,8 +296,8 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
             for (int c = 0; c < 2; ++c)
                 for (int k = 0; k < 4; ++k) a.in[c][k] = p->lv[src_level].msgs[c][k];
             a.src = src_level == t ? kSrcSame : kSrcCoarse;
-            a.stride = g.nplane * G;
-            a.in_stride = p->geom[src_level].nplane * G;
+            a.stride = a.in_stride = (int64_t)ginf::kTileSlots * G;
+            a.tile = (int64_t)C * ginf::kTileSlots * G;
             a.w = g.w; a.h = g.h; a.whp = g.whp; a.nslots = g.nslots;
             a.in_whp = p->geom[src_level].whp;
             a.color = color;
@@ -316,7 +310,8 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
             // byte view for the quad kernel
             a.arr_bytes = (uint32_t)p->arr_bytes[t];
             a.data_bytes = (uint32_t)p->arr_bytes[t];
-            a.plane_bytes = (uint32_t)(g.nplane * G * 4);
+            a.plane_bytes = (uint32_t)(ginf::kTileSlots * G * 4);
+            a.tile_bytes = (uint32_t)(a.tile * 4);
             a.data_plane_bytes = a.plane_bytes / 2;
             a.fit_flag = d16 ? p->flag_dev : nullptr;
             a.out_base = p->lv[t].msgs[color][0];
@@ -328,7 +323,7 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
             } else {
                 a.in_base = p->lv[src_level].msgs[0][0];
                 a.in_arr_bytes = (uint32_t)p->arr_bytes[src_level];
-                a.in_plane_bytes = (uint32_t)(p->geom[src_level].nplane * G * 4);
+                a.in_plane_bytes = a.plane_bytes;
                 a.in_span = (uint32_t)(8 * p->arr_bytes[src_level]);
             }
             const bool labels = (t == 0) && (i >= N - 2);
@@ -419,7 +414,7 @@ int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, gin
         g.w = w; g.h = h;
         g.whp = (int)round_up((w + 1) / 2, 16);
         g.nslots = g.whp * h;
-        g.nplane = round_up(g.nslots, 64) + (g.nslots >= 65536 ? GINF_PLANE_PAD : 0);
+        g.nplane = round_up(g.nslots, ginf::kTileSlots);
         p->geom.push_back(g);
         const ginf::StateLayout ly = ginf::state_layout(sh, L);
         const size_t arr = (size_t)ly.C * g.nplane * ly.G * es;
@@ -611,7 +606,7 @@ int ginf_bp_plan_dump_msgs(ginf_bp_plan p, int level, void *msgs_host) {
                         const int q = ly.interleave ? l % G : l / C;
                         const int i = ly.interleave ? l / G : l - q * C;
                         dst[(size_t)x + (size_t)g.w * ((size_t)y + (size_t)g.h * ((size_t)d + 4u * l))] =
-                            buf[((size_t)i * g.nplane + slot) * G + q];
+                            buf[ginf::elem_index(slot, i, q, C, G)];
                     }
                 }
         }
@@ -654,7 +649,7 @@ int ginf_bp_plan_dump_data(ginf_bp_plan p, int level, void *data_host) {
                     const int q = ly.interleave ? l % G : l / C;
                     const int i = ly.interleave ? l / G : l - q * C;
                     dst[(size_t)x + (size_t)g.w * ((size_t)y + (size_t)g.h * (size_t)l)] =
-                        buf[((size_t)i * g.nplane + slot) * G + q];
+                        buf[ginf::elem_index(slot, i, q, C, G)];
                 }
             }
     }
@@ -900,7 +895,8 @@ int ginf_batch_create(int n, const ginf_bp_desc *descs, const void *const *table
             const ginf::StateLayout ly = ginf::state_layout(p->shape, d.num_labels);
             a.data = p->lv[0].data[c];
             for (int k = 0; k < 4; ++k) { a.out[k] = p->lv[0].msgs[c][k]; a.in[0][k] = p->lv[0].msgs[0][k]; a.in[1][k] = p->lv[0].msgs[1][k]; }
-            a.stride = a.in_stride = g.nplane * ly.G;
+            a.stride = a.in_stride = (int64_t)ginf::kTileSlots * ly.G;
+            a.tile = (int64_t)ly.C * ginf::kTileSlots * ly.G;
             a.w = g.w; a.h = g.h; a.whp = g.whp; a.nslots = g.nslots; a.in_whp = g.whp;
             a.color = c;
             a.src = ginf::kSrcSame;
@@ -910,7 +906,8 @@ int ginf_batch_create(int n, const ginf_bp_desc *descs, const void *const *table
             a.s_f = p->s_f; a.tau_f = p->tau_f; a.s_i = p->s_i; a.tau_i = p->tau_i;
             a.table = p->table_dev;
             a.arr_bytes = a.data_bytes = a.in_arr_bytes = (uint32_t)p->arr_bytes[0];
-            a.plane_bytes = a.in_plane_bytes = (uint32_t)(g.nplane * ly.G * 4);
+            a.plane_bytes = a.in_plane_bytes = (uint32_t)(ginf::kTileSlots * ly.G * 4);
+            a.tile_bytes = (uint32_t)(a.tile * 4);
             a.data_plane_bytes = a.plane_bytes / 2;
             a.fit_flag = p->flag_dev;
             a.out_base = p->lv[0].msgs[c][0];

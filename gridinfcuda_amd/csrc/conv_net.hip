@@ -281,12 +281,12 @@ struct InjectArgs {
 // One thread per (register plane, slot) of the outbox array of colour
 // blockIdx.y & 1 and direction blockIdx.y >> 1 (plane blockIdx.z), writing
 // that slot's G labels contiguously (a 16-byte store for the quad layout), so
-// the writes of a wave are one contiguous run.  Border nodes get 0; padding
+// the writes of a wave (64 slots of one slot tile) are one contiguous run.  Border nodes get 0; padding
 // slots and padding labels are left as they are.
 __global__ void k_inject(const float *__restrict__ pred, InjectArgs a) {
     const int c = blockIdx.y & 1, d = blockIdx.y >> 1;
     const int ii = blockIdx.z;
-    float *__restrict__ out = (float *)a.msgs[c][d] + (int64_t)ii * a.g.nplane * a.G;
+    float *__restrict__ out = (float *)a.msgs[c][d];
     const int G = a.G, C = a.C, L = a.L;
     for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < a.g.nslots; slot += gridDim.x * blockDim.x) {
         const int y = slot / a.g.whp;
@@ -296,7 +296,7 @@ __global__ void k_inject(const float *__restrict__ pred, InjectArgs a) {
         const bool interior = x >= 1 && y >= 1 && x <= a.g.w - 2 && y <= a.g.h - 2;
         const int X = min(x >> a.shift, a.pw - 1), Y = min(y >> a.shift, a.ph - 1);
         const float *p = pred + ((int64_t)Y * a.pw + X) * 4 * L + d * L;
-        float *o = out + (int64_t)slot * G;
+        float *o = out + ginf::elem_index(slot, ii, 0, C, G);
         if (a.interleave && G == 4 && ii * 4 + 3 < L) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (interior) {
@@ -332,7 +332,7 @@ __global__ void k_block_mean_state(InjectArgs a, int f, float scale, float *__re
             for (int x = X * f; x < X * f + f && x < a.g.w; ++x) {
                 const int c = (x + y) & 1;
                 const int64_t slot = (int64_t)y * a.g.whp + (x >> 1);
-                acc += ((const float *)a.msgs[c][d])[((int64_t)ii * a.g.nplane + slot) * a.G + q];
+                acc += ((const float *)a.msgs[c][d])[ginf::elem_index(slot, ii, q, a.C, a.G)];
                 ++cnt;
             }
         out[i] = cnt ? acc / cnt * scale : 0.f;

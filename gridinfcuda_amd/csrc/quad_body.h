@@ -40,9 +40,11 @@ __device__ __forceinline__ int quad_bcast(int v) {
 //         of every level is an integer in [0, 65535]) and a lane loads its 4
 //         labels' costs as 8 bytes; otherwise they were stored in T.  The flag
 //         is uniform, so the branch never diverges.
-// Layout: label quads, element (l, slot) at ((l/4) * nplane + slot) * 4 + l%4,
-// so one lane moves 4 labels (16 B) per access and 16 consecutive nodes of a
-// wave cover 256 contiguous bytes of every array.  Offsets are 32-bit and
+// Layout: label quads in slot tiles, element (l, slot) at
+// elem_index(slot, l/4, l%4, C, 4): one lane moves 4 labels (16 B) per
+// access, 16 consecutive nodes of a wave cover 256 contiguous bytes of every
+// array, and the next label quad of the same nodes lies 1 KiB further on.
+// Offsets are 32-bit and
 // carried entirely in the VGPR offset (soffset = 0): with the plane offset in
 // soffset, the compiler reused SGPRs that a v_cmp in the labelling code had
 // just written, and the memory instruction then intermittently read the wrong
@@ -62,8 +64,8 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
     const __amdgpu_buffer_rsrc_t drs = __builtin_amdgcn_make_buffer_rsrc((void *)a.data, 0, a.data_bytes, 0x00020000);
     const __amdgpu_buffer_rsrc_t irs = __builtin_amdgcn_make_buffer_rsrc((void *)a.in_base, 0, a.in_span, 0x00020000);
     const __amdgpu_buffer_rsrc_t ors = __builtin_amdgcn_make_buffer_rsrc(a.out_base, 0, 4u * a.arr_bytes, 0x00020000);
-    const uint32_t self_off = (uint32_t)slot * 16u;
-    const uint32_t data_off = use16 ? (uint32_t)slot * 8u : self_off;
+    const uint32_t self_off = (uint32_t)(slot >> kTileShift) * a.tile_bytes + (uint32_t)(slot & (kTileSlots - 1)) * 16u;
+    const uint32_t data_off = use16 ? self_off / 2u : self_off;  // 16-bit storage: every offset halves
     const uint32_t data_plane = use16 ? a.data_plane_bytes : a.plane_bytes;
     uint32_t in_off = 0;
     if (!ZEROIN) {
@@ -72,7 +74,9 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
         if (a.src == kSrcCoarse) { nx >>= 1; ny >>= 1; csel = (nx + ny) & 1; }
         else csel = 0;  // in_base already points at the partner colour's arrays
         const int od = (d + 2) & 3;
-        in_off = (uint32_t)(csel * 4 + od) * a.in_arr_bytes + (uint32_t)(ny * a.in_whp + (nx >> 1)) * 16u;
+        const uint32_t ns = (uint32_t)(ny * a.in_whp + (nx >> 1));
+        in_off = (uint32_t)(csel * 4 + od) * a.in_arr_bytes + (ns >> kTileShift) * a.tile_bytes +
+                 (ns & (kTileSlots - 1)) * 16u;
     }
     const uint32_t out_off = (uint32_t)d * a.arr_bytes + self_off;
 

@@ -56,6 +56,12 @@ __device__ __forceinline__ T lane_down(T v, int off) { return __shfl_down(v, off
 template <int G, typename T>
 __device__ __forceinline__ T lane_xor(T v, int m) { return __shfl_xor(v, m, G); }
 
+// Element of (slot n, plane 0, lane 0) of a state array of the plan
+// (slot_elem of bp_layout.h with the tile size taken from the arguments).
+__device__ __forceinline__ int64_t tile_slot(const SweepArgs &a, int64_t n, int G) {
+    return (n >> kTileShift) * a.tile + (n & (kTileSlots - 1)) * G;
+}
+
 // Locate the node this lane group works on.  Returns false for slots that
 // are padding, border nodes or beyond the level (those never move).
 __device__ __forceinline__ bool locate(const SweepArgs &a, int G, int &slot, int &q, int &x, int &y) {
@@ -82,7 +88,7 @@ __device__ __forceinline__ void incoming_ptrs(const SweepArgs &a, int x, int y, 
         const int od = (d + 2) & 3;
         const int c = (nx + ny) & 1;
         const T *base = (const T *)(c ? a.in[1][od] : a.in[0][od]);
-        ip[d] = base + ((int64_t)ny * a.in_whp + (nx >> 1)) * G + q;
+        ip[d] = base + tile_slot(a, (int64_t)ny * a.in_whp + (nx >> 1), G) + q;
     }
 }
 

@@ -12,7 +12,7 @@ __global__ __launch_bounds__(256) void k_sweep_general(const SweepArgs a) {
     if (!locate(a, 1, slot, q, x, y)) return;
     const int L = a.num_labels;
     const int64_t stride = a.stride;
-    const int64_t self = slot;
+    const int64_t self = tile_slot(a, slot, 1);
     const T *dp = (const T *)a.data + self;
     const T *ip[4];
     if (!ZEROIN) incoming_ptrs<T, 1>(a, x, y, 0, ip);
