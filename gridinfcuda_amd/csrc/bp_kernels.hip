@@ -292,16 +292,21 @@ __global__ __launch_bounds__(256) void k_energy_final(const double *partials, in
 // (slot k of row y in the node's colour) and, fused, the level-1 node (k, Y)
 // as the child sum in the reference order (2X,2Y), (2X+1,2Y), (2X,2Y+1),
 // (2X+1,2Y+1).  Labels past L are stored as 0, as the tiled kernel does.
+// Planes i = blockIdx.z, blockIdx.z + gridDim.z, ... < C: the plan launches
+// one z-block per plane, and the conditional redo one z-block in all, so that
+// the redo's blocks are few when they have nothing to do.
 template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src, int W, int H, int L, LevelGeom g0,
-                                                       S *d00, S *d01, int coarse, LevelGeom g1, S *d10, S *d11,
-                                                       int *flag, const int *only_if) {
+__global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src, int W, int H, int L, int C,
+                                                       LevelGeom g0, S *d00, S *d01, int coarse, LevelGeom g1, S *d10,
+                                                       S *d11, int *flag, const int *only_if) {
     if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
     const int k = blockIdx.x * blockDim.x + threadIdx.x;  // column pair
-    const int Y = blockIdx.y, i = blockIdx.z, C = gridDim.z;  // one z-block per plane
+    const int Y = blockIdx.y;
     const int x0 = 2 * k, y0 = 2 * Y;
     if (x0 >= W) return;
     const bool ex = x0 + 1 < W, ey = y0 + 1 < H;
+    bool bad = false;
+    for (int i = blockIdx.z; i < C; i += gridDim.z) {
     T v[4][2][2];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -317,7 +322,6 @@ __global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src
             }
         }
     }
-    bool bad = false;
     typedef S s4 __attribute__((ext_vector_type(4)));
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
@@ -350,6 +354,47 @@ __global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src
         }
         *(s4 *)(dst + elem_index(slot, i, 0, C, 4)) = o;
     }
+    }
+    report_bad(bad, flag);
+}
+
+// Data-cost pyramid level for the label-quad layout: the rule of k_pyramid,
+// one thread per (plane i, coarse node) moving the node's label quad as one
+// vector per child (8 B in 16-bit storage, 16 B in T), grid-stride.
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_pyramid_quad(const S *f0, const S *f1, LevelGeom fg, S *c0, S *c1,
+                                                      LevelGeom cg, int C, int *flag, const int *only_if) {
+    if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
+    typedef S s4 __attribute__((ext_vector_type(4)));
+    const int64_t nodes = (int64_t)cg.w * cg.h;
+    const int64_t total = nodes * C;
+    bool bad = false;
+    for (int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+         gid += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(gid / nodes);
+        const int node = (int)(gid - (int64_t)i * nodes);
+        const int X = node % cg.w, Y = node / cg.w;
+        const bool ex = 2 * X + 1 < fg.w, ey = 2 * Y + 1 < fg.h;
+        const int64_t e00 = elem_index((int64_t)(2 * Y) * fg.whp + X, i, 0, C, 4);
+        const int64_t e01 = elem_index((int64_t)(2 * Y + 1) * fg.whp + X, i, 0, C, 4);
+        const s4 v00 = *(const s4 *)(f0 + e00);
+        s4 v10 = s4(0), v01 = s4(0), v11 = s4(0);
+        if (ex) v10 = *(const s4 *)(f1 + e00);
+        if (ey) v01 = *(const s4 *)(f1 + e01);
+        if (ex && ey) v11 = *(const s4 *)(f0 + e01);
+        s4 o;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            T acc = T(0);
+            acc = acc + from_store<S, T>(v00[q]);
+            if (ex) acc = acc + from_store<S, T>(v10[q]);
+            if (ey) acc = acc + from_store<S, T>(v01[q]);
+            if (ex && ey) acc = acc + from_store<S, T>(v11[q]);
+            o[q] = to_store<S>(acc, bad);
+        }
+        S *dst = ((X + Y) & 1) ? c1 : c0;
+        *(s4 *)(dst + elem_index((int64_t)Y * cg.whp + (X >> 1), i, 0, C, 4)) = o;
+    }
     report_bad(bad, flag);
 }
 
@@ -381,11 +426,12 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
                                const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
                                void *d11, hipStream_t st, bool data16, int *flag, const int *only_if) {
     if (ly.interleave && ly.G == 4) {  // label quads: register-only relayout
-        const dim3 grid((unsigned)((W + 511) / 512), (unsigned)((H + 1) / 2), (unsigned)ly.C), block(256);
+        const dim3 grid((unsigned)((W + 511) / 512), (unsigned)((H + 1) / 2), only_if ? 1u : (unsigned)ly.C),
+            block(256);
         if (ly.C > 65535 || (H + 1) / 2 > 65535) return hipErrorInvalidValue;
-#define GINF_RQ(TT, SS)                                                                                           \
-        hipLaunchKernelGGL((k_relayout_quad<TT, SS>), grid, block, 0, st, (const TT *)src, W, H, L, g0, (SS *)d00, \
-                           (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, only_if)
+#define GINF_RQ(TT, SS)                                                                                          \
+        hipLaunchKernelGGL((k_relayout_quad<TT, SS>), grid, block, 0, st, (const TT *)src, W, H, L, ly.C, g0,     \
+                           (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, only_if)
         if (data16) { if (dtype == 0) GINF_RQ(float, uint16_t); else GINF_RQ(int, uint16_t); }
         else { if (dtype == 0) GINF_RQ(float, float); else GINF_RQ(int, int); }
 #undef GINF_RQ
@@ -413,6 +459,22 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
                           const LevelGeom &cg, int C, int G, hipStream_t st, bool data16, int *flag,
                           const int *only_if) {
+    if (G == 4) {
+        // four lanes per node: one vector per child (the sums are per element,
+        // so the label order of the layout does not matter)
+        const int64_t threads = (int64_t)cg.w * cg.h * C;
+        int64_t blocks = (threads + 255) / 256;
+        const int64_t cap = only_if ? 1024 : 16384;  // the redo usually exits at once
+        if (blocks > cap) blocks = cap;
+        const dim3 grid((unsigned)blocks), block(256);
+#define GINF_PQ(TT, SS)                                                                                           \
+        hipLaunchKernelGGL((k_pyramid_quad<TT, SS>), grid, block, 0, st, (const SS *)f0, (const SS *)f1, fg,       \
+                           (SS *)c0, (SS *)c1, cg, C, flag, only_if)
+        if (data16) { if (dtype == 0) GINF_PQ(float, uint16_t); else GINF_PQ(int, uint16_t); }
+        else { if (dtype == 0) GINF_PQ(float, float); else GINF_PQ(int, int); }
+#undef GINF_PQ
+        return hipGetLastError();
+    }
     const int64_t threads = (int64_t)cg.w * cg.h * G * C;
     const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
 #define GINF_PY(TT, SS)                                                                                            \

@@ -6,15 +6,15 @@
 // where whp >= ceil(w / 2) is the padded half-row.  A node's L labels are
 // spread over G lanes ("lanes per node"): lane q owns labels q*C .. q*C+C-1
 // (register plane i holds label q*C + i, or 4i + q for the label-quad layout).
-// Slots are grouped in tiles of kTileSlots = 64; an array of C planes stores
-// tile-major, then plane, then slot within the tile, then lane:
-//   element(n, i, q) = ((n / 64) * C + i) * 64 * G + (n % 64) * G + q.
+// Slots are grouped in tiles of kTileSlots (S = 256) slots; an array of C
+// planes stores tile-major, then plane, then slot within the tile, then lane:
+//   element(n, i, q) = ((n / S) * C + i) * S * G + (n % S) * G + q.
 // One wavefront reading plane i of 64/G consecutive nodes touches 64
 // consecutive elements (256 B) of every array, and a wave's successive planes
-// follow each other at 64*G elements, so a half-sweep reads each array as a
+// follow each other at S*G elements, so a half-sweep reads each array as a
 // few long runs instead of C runs one plane (nplane slots) apart.  Measured on
-// the 1920x1080x128 half-sweep: 5.54-5.61 TB/s for tiles of 64-1024 slots
-// against 5.18 TB/s for plane-major arrays.
+// the 1920x1080x128 decode (5 levels x 10 half-sweeps): 11.61 ms with S = 256,
+// 11.75 ms with S = 64, 11.84 ms with S = 1024, 12.17 ms plane-major.
 //
 // Per level and colour there is one data-cost array and four outbox arrays
 // (directions S, E, N, W as in src/include/utils.h:26-37).  Outbox (x,y,d) is
@@ -31,7 +31,11 @@ constexpr int kNumDir = 4;
 constexpr int kSrcSame = 0;    // incoming messages from this level, other colour
 constexpr int kSrcCoarse = 1;  // incoming messages read through the parent at level+1
 
-constexpr int kTileShift = 6;
+// log2 of the slots per tile.
+#ifndef GINF_TILE_SHIFT
+#define GINF_TILE_SHIFT 8
+#endif
+constexpr int kTileShift = GINF_TILE_SHIFT;
 constexpr int kTileSlots = 1 << kTileShift;
 
 struct LevelGeom {

@@ -43,7 +43,8 @@ __device__ __forceinline__ int quad_bcast(int v) {
 // Layout: label quads in slot tiles, element (l, slot) at
 // elem_index(slot, l/4, l%4, C, 4): one lane moves 4 labels (16 B) per
 // access, 16 consecutive nodes of a wave cover 256 contiguous bytes of every
-// array, and the next label quad of the same nodes lies 1 KiB further on.
+// array, and the next label quad of the same nodes lies kTileSlots * 16 B
+// further on.
 // Offsets are 32-bit and
 // carried entirely in the VGPR offset (soffset = 0): with the plane offset in
 // soffset, the compiler reused SGPRs that a v_cmp in the labelling code had
