@@ -87,20 +87,22 @@ hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const Level
                           const LevelGeom &cg, int C, int G, hipStream_t st, bool data16 = false, int *flag = nullptr,
                           const int *only_if = nullptr);
 hipError_t launch_upsample(int dtype, const UpsampleArgs &u, hipStream_t st);
-// Batched persistent half-sweep (sweep_batch.hip): args[i] describes instance
-// i for the moving colour; items[k] = (instance, first slot, slots).
+// Batched persistent half-sweep (sweep_batch.hip) over instances of label
+// capacity lmax (16, 32, 64, 96 or 128), all with L == lmax when `full`:
+// args[i] describes instance i for the moving colour; items[k] = (instance,
+// first slot, slots).
 struct BatchItem {
     int inst;
     int slot0;
     int nslots;
     int pad;
 };
-hipError_t launch_sweep_batch(int dtype, const SweepArgs *args_dev, const BatchItem *items_dev, int nitems, int grid,
-                              bool zeroin, bool labels, hipStream_t st);
+hipError_t launch_sweep_batch(int dtype, int lmax, bool full, const SweepArgs *args_dev, const BatchItem *items_dev, int nitems,
+                              int grid, bool zeroin, bool labels, hipStream_t st);
 // The same with each instance's data costs in 16-bit storage when its
 // SweepArgs::fit_flag reads 0 (sweep_batch16.hip).
-hipError_t launch_sweep_batch16(int dtype, const SweepArgs *args_dev, const BatchItem *items_dev, int nitems, int grid,
-                                bool zeroin, bool labels, hipStream_t st);
+hipError_t launch_sweep_batch16(int dtype, int lmax, bool full, const SweepArgs *args_dev, const BatchItem *items_dev, int nitems,
+                                int grid, bool zeroin, bool labels, hipStream_t st);
 
 // Seed a level's outboxes from an NHWC prediction (conv_net.hip).
 hipError_t launch_inject(const float *pred, void *const msgs[2][4], const LevelGeom &g, int C, int G, bool interleave,

@@ -726,10 +726,12 @@ int ginf_energy_f64(int width, int height, int num_labels, int dtype, const void
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Batches: many single-level instances advanced together.  Each half-sweep
-// is one persistent launch over a work list covering every instance; the
-// whole decode (relayout + all half-sweeps) is captured once into a HIP graph
-// and replayed.
+// Batches: many single-level instances advanced together.  The work list is
+// grouped by label capacity; each group's half-sweeps are persistent launches
+// of the kernel built for that capacity, on a graph branch of their own (the
+// groups share no state, so they run concurrently and only join at the end).
+// The whole decode (relayouts + all half-sweeps) is captured once into a HIP
+// graph and replayed.
 
 struct ginf_batch_s {
     int n = 0;
@@ -739,6 +741,15 @@ struct ginf_batch_s {
     ginf::SweepArgs *args_dev = nullptr;  // [2][n]
     ginf::BatchItem *items_dev = nullptr;
     int nitems = 0;
+    // work-list groups by label capacity and L == capacity or not: items
+    // [off, off + count) of items_dev
+    static constexpr int kMaxGroups = 10;  // capacities 16, 32, 64, 96, 128, full or not
+    int ngroups = 0;
+    int group_lmax[kMaxGroups] = {}, group_off[kMaxGroups] = {}, group_count[kMaxGroups] = {};
+    bool group_full[kMaxGroups] = {};
+    std::vector<int> inst_group;  // group of each instance
+    hipStream_t group_stream[kMaxGroups] = {};  // capture-only streams of the group branches
+    hipEvent_t fork_ev = nullptr, join_ev[kMaxGroups] = {};
     int slots_per_item[4] = {256, 256, 256, 256};  // by label capacity 16, 32, 64, 128
     int blocks_per_cu = 4;
     int num_cu = 0;
@@ -772,13 +783,39 @@ int batch_build_items(ginf_batch b) {
         const int per = b->slots_per_item[lmax_class(p->shape.C)];
         for (int s0 = 0; s0 < ns; s0 += per) items.push_back({i, s0, ns - s0 < per ? ns - s0 : per, 0});
     }
-    // Largest instances' items interleaved with the rest: order by item index
-    // within instance so every instance is in flight early.
+    // Grouped by label capacity; within a group the largest instances' items
+    // are interleaved with the rest (ordered by item index within the
+    // instance) so that every instance is in flight early.
+    auto key = [&](int inst) {  // capacity, then full instances first
+        const ginf_bp_plan p = b->plans[inst];
+        return 2 * p->shape.C + (p->desc.num_labels == p->shape.C ? 0 : 1);
+    };
     std::stable_sort(items.begin(), items.end(), [&](const ginf::BatchItem &x, const ginf::BatchItem &y) {
-        const int rx = x.slot0 / b->slots_per_item[lmax_class(b->plans[x.inst]->shape.C)];
-        const int ry = y.slot0 / b->slots_per_item[lmax_class(b->plans[y.inst]->shape.C)];
+        const int kx = key(x.inst), ky = key(y.inst);
+        if (kx != ky) return kx < ky;
+        const int lx = b->plans[x.inst]->shape.C, ly = b->plans[y.inst]->shape.C;
+        const int rx = x.slot0 / b->slots_per_item[lmax_class(lx)];
+        const int ry = y.slot0 / b->slots_per_item[lmax_class(ly)];
         return rx < ry;
     });
+    b->ngroups = 0;
+    int last = -1;
+    for (size_t k = 0; k < items.size(); ++k) {
+        const int kk = key(items[k].inst);
+        if (kk != last) {
+            if (b->ngroups == ginf_batch_s::kMaxGroups) return set_error(GINF_ERR_UNSUPPORTED, "too many label capacities");
+            last = kk;
+            b->group_lmax[b->ngroups] = kk / 2;
+            b->group_full[b->ngroups] = (kk & 1) == 0;
+            b->group_off[b->ngroups] = (int)k;
+            b->group_count[b->ngroups] = 0;
+            ++b->ngroups;
+        }
+        ++b->group_count[b->ngroups - 1];
+    }
+    b->inst_group.assign(b->n, 0);
+    for (int g = 0; g < b->ngroups; ++g)
+        for (int k = b->group_off[g]; k < b->group_off[g] + b->group_count[g]; ++k) b->inst_group[items[k].inst] = g;
     if (b->items_dev) { hipFree(b->items_dev); b->items_dev = nullptr; }
     b->nitems = (int)items.size();
     GINF_HIP_TRY(hipMalloc(&b->items_dev, sizeof(ginf::BatchItem) * (items.empty() ? 1 : items.size())));
@@ -798,26 +835,36 @@ int batch_capture(ginf_batch b, const void *const *data_dev, int32_t *const *lab
     GINF_HIP_TRY(hipMemcpy(b->args_dev + b->n, b->args_host[1].data(), sizeof(SweepArgs) * b->n, hipMemcpyHostToDevice));
     const int N = b->iters;
     hipStream_t st = b->cap;
+    for (int k = 0; k < b->ngroups; ++k) {  // branch streams, created on first use
+        if (!b->group_stream[k]) GINF_HIP_TRY(hipStreamCreateWithFlags(&b->group_stream[k], hipStreamNonBlocking));
+        if (!b->join_ev[k]) GINF_HIP_TRY(hipEventCreateWithFlags(&b->join_ev[k], hipEventDisableTiming));
+    }
     GINF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = GINF_OK;
     auto fail = [&](hipError_t e) {
         hipGraph_t g = nullptr;
         hipStreamEndCapture(st, &g);
         if (g) hipGraphDestroy(g);
         return set_error(GINF_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
     };
-    for (int i = 0; i < b->n && rc == GINF_OK; ++i) {
+    hipError_t e = hipEventRecord(b->fork_ev, st);
+    if (e != hipSuccess) return fail(e);
+    for (int k = 0; k < b->ngroups; ++k)
+        if ((e = hipStreamWaitEvent(b->group_stream[k], b->fork_ev, 0)) != hipSuccess) return fail(e);
+    // Each instance's data costs (and, for fewer than two half-sweeps, its
+    // zero fills and labels) on its group's branch, so that one group's
+    // relayouts overlap the other groups' half-sweeps.
+    for (int i = 0; i < b->n; ++i) {
         const ginf_bp_plan p = b->plans[i];
+        hipStream_t gs = b->group_stream[b->inst_group[i]];
         const StateLayout ly = state_layout(p->shape, p->desc.num_labels);
         auto relayout = [&](bool d16, int *flag, const int *only_if) {
             return launch_relayout_pyr(b->dtype, data_dev[i], p->desc.width, p->desc.height, p->desc.num_labels, ly,
                                        p->geom[0], p->lv[0].data[0], p->lv[0].data[1], false, p->geom[0],
-                                       p->lv[0].data[0], p->lv[0].data[1], st, d16, flag, only_if);
+                                       p->lv[0].data[0], p->lv[0].data[1], gs, d16, flag, only_if);
         };
-        hipError_t e;
         if (b->compact) {
             // 16-bit costs where the instance's costs fit, else redone in T (decided on the device)
-            if ((e = launch_fill_u32(p->flag_dev, 1, 0u, st)) != hipSuccess) return fail(e);
+            if ((e = launch_fill_u32(p->flag_dev, 1, 0u, gs)) != hipSuccess) return fail(e);
             if ((e = relayout(true, p->flag_dev, nullptr)) != hipSuccess) return fail(e);
             if ((e = relayout(false, nullptr, p->flag_dev)) != hipSuccess) return fail(e);
         } else if ((e = relayout(false, nullptr, nullptr)) != hipSuccess) {
@@ -828,21 +875,27 @@ int batch_capture(ginf_batch b, const void *const *data_dev, int32_t *const *lab
                 const int64_t per = (int64_t)ly.C * p->geom[0].nplane * ly.G;
                 for (int c = 0; c < 2; ++c)
                     for (int k = 0; k < 4; ++k)
-                        if ((e = launch_fill_u32(p->lv[0].msgs[c][k], per, 0u, st)) != hipSuccess) return fail(e);
+                        if ((e = launch_fill_u32(p->lv[0].msgs[c][k], per, 0u, gs)) != hipSuccess) return fail(e);
             }
-            for (int c = (N == 1 ? 0 : 0); c < (N == 1 ? 1 : 2); ++c)
-                if ((e = launch_zero_labels(labels_dev[i], p->desc.width, p->desc.height, c, st)) != hipSuccess)
+            for (int c = 0; c < (N == 1 ? 1 : 2); ++c)
+                if ((e = launch_zero_labels(labels_dev[i], p->desc.width, p->desc.height, c, gs)) != hipSuccess)
                     return fail(e);
         }
     }
     const int grid = b->num_cu * b->blocks_per_cu;
-    for (int it = 0; it < N; ++it) {
-        const int color = (it + 1) & 1;
-        const bool zeroin = N >= 2 && it == 0 && !b->seeded;
-        const bool labels = it >= N - 2;
-        hipError_t e = (b->compact ? launch_sweep_batch16 : launch_sweep_batch)(
-            b->dtype, b->args_dev + (size_t)color * b->n, b->items_dev, b->nitems, grid, zeroin, labels, st);
-        if (e != hipSuccess) return fail(e);
+    for (int k = 0; k < b->ngroups; ++k) {
+        hipStream_t gs = b->group_stream[k];
+        for (int it = 0; it < N; ++it) {
+            const int color = (it + 1) & 1;
+            const bool zeroin = N >= 2 && it == 0 && !b->seeded;
+            const bool labels = it >= N - 2;
+            e = (b->compact ? launch_sweep_batch16 : launch_sweep_batch)(
+                b->dtype, b->group_lmax[k], b->group_full[k], b->args_dev + (size_t)color * b->n,
+                b->items_dev + b->group_off[k], b->group_count[k], grid, zeroin, labels, gs);
+            if (e != hipSuccess) return fail(e);
+        }
+        if ((e = hipEventRecord(b->join_ev[k], gs)) != hipSuccess) return fail(e);
+        if ((e = hipStreamWaitEvent(st, b->join_ev[k], 0)) != hipSuccess) return fail(e);
     }
     hipGraph_t g = nullptr;
     GINF_HIP_TRY(hipStreamEndCapture(st, &g));
@@ -925,6 +978,8 @@ int ginf_batch_create(int n, const ginf_bp_desc *descs, const void *const *table
     if (e != hipSuccess) return cleanup(set_error(GINF_ERR_NOMEM, "batch args: %s", hipGetErrorString(e)));
     e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(set_error(GINF_ERR_HIP, "batch stream: %s", hipGetErrorString(e)));
+    e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming);
+    if (e != hipSuccess) return cleanup(set_error(GINF_ERR_HIP, "batch event: %s", hipGetErrorString(e)));
     int rc = batch_build_items(b);
     if (rc) return cleanup(rc);
     *out = b;
@@ -999,6 +1054,11 @@ int ginf_batch_destroy(ginf_batch b) {
     if (!b) return GINF_OK;
     batch_drop_graph(b);
     if (b->cap) hipStreamDestroy(b->cap);
+    for (int k = 0; k < ginf_batch_s::kMaxGroups; ++k) {
+        if (b->group_stream[k]) hipStreamDestroy(b->group_stream[k]);
+        if (b->join_ev[k]) hipEventDestroy(b->join_ev[k]);
+    }
+    if (b->fork_ev) hipEventDestroy(b->fork_ev);
     if (b->args_dev) hipFree(b->args_dev);
     if (b->items_dev) hipFree(b->items_dev);
     for (auto p : b->plans) ginf_bp_plan_destroy(p);
