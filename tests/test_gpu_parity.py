@@ -239,6 +239,21 @@ def test_host_entry_reuses_state_correctly(oracle_mod):
     np.testing.assert_array_equal(G.hbp_host(c, tc, 2, 2, 5), oracle_mod.hbp(c, tc, 2, 2, 5))
 
 
+def test_host_entry_large_upload(oracle_mod):
+    """A cost volume of several upload chunks (100.7 MB, not a multiple of
+    the 32 MiB pinned staging buffer) arrives intact: the host entry's labels
+    equal the device-resident plan run's (whose parity the other tests pin),
+    for two different volumes of the same shape in a row."""
+    say("host entry large upload")
+    L, H, W = 97, 401, 647
+    table = oracle_mod.table_trunc_linear(L, 8, 32)
+    for seed in (51, 52):
+        data = stereo(seed, W, H, L, 16, L - 1, 40, np.float32)
+        want, _, _ = run_plan(data, table, 2, 4, 6, norm=0)
+        got = G.hbp_host(data, table, 2, 4, 6, norm=0)
+        np.testing.assert_array_equal(got, want, err_msg=f"seed {seed}")
+
+
 # ---------------------------------------------------------------- 16-bit data costs
 
 def _pyramid_ref(oracle_mod, data, levels):
