@@ -106,6 +106,31 @@ def test_linear_bit_exact(oracle_mod, H, W, L, lanes, levels, iters, kind, dt, n
     np.testing.assert_array_equal(msgs, ref_msgs)
 
 
+@pytest.mark.parametrize("H,W,L,kind,kernel,lanes,dt,norm", [
+    (70, 530, 32, "tl", 2, 0, "float32", 0),     # quad; half-row 272 slots: 256-slot tiles straddle rows
+    (67, 1001, 22, "potts", 2, 0, "int32", 0),   # quad, L not a multiple of 4; half-row 512 = two tiles
+    (45, 700, 24, "tl", 1, 4, "int32", 1),       # chunked, four lanes per node
+    (29, 611, 9, "tq", 0, 0, "float32", 0),      # general O(L^2) kernel
+])
+def test_slot_tiles_across_rows(oracle_mod, H, W, L, kind, kernel, lanes, dt, norm):
+    """Grids whose half-rows are not multiples of the 256-slot tile (so tiles
+    straddle rows and the last tile of a level is partial) and whose coarse
+    levels are smaller than one tile: labels and level-0 messages equal the
+    oracle's bit for bit."""
+    say(f"tiles across rows H={H} W={W} L={L} {kind} kernel={kernel} lanes={lanes} {dt} norm={norm}")
+    rng = np.random.default_rng(zlib.crc32(repr((H, W, L, kind, kernel)).encode()))
+    dtype = np.dtype(dt)
+    data = rng.integers(0, 40, size=(L, H, W)).astype(dtype)
+    table = table_for(oracle_mod, kind, L, dtype)
+    init = rng.integers(0, L, size=(H, W)).astype(np.int32)
+    lab, msgs, _ = run_plan(data, table, SM[kind], 4, 4, norm=norm, lanes=lanes, labels_init=init, dump_level=0,
+                            kernel=kernel)
+    ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, SM[kind], 4, 4, norm=norm, labels_init=init,
+                                          return_state=True)
+    np.testing.assert_array_equal(lab, ref_lab)
+    np.testing.assert_array_equal(msgs, ref_msgs)
+
+
 @pytest.mark.parametrize("kind", ["tq", "explicit"])
 @pytest.mark.parametrize("dt", ["int32", "float32"])
 @pytest.mark.parametrize("L,levels,iters", [(5, 1, 4), (16, 2, 3), (33, 3, 5), (128, 2, 2)])
