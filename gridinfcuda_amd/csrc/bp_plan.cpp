@@ -208,11 +208,18 @@ hipError_t upload_pageable(HostCache &hc, void *dst_dev, const void *src_host, s
         if (nth == 1 || len < ((size_t)1 << 20)) {
             std::memcpy(dst, src, len);
         } else {
+            // slices 1..nth-1 on helper threads, slice 0 here; a slice whose
+            // thread cannot be started is copied here too (no exception may
+            // cross the C ABI)
             const size_t per = (len / nth + 4095) & ~(size_t)4095;
             std::vector<std::thread> th;
             for (int t = 1; t < nth && (size_t)t * per < len; ++t) {
                 const size_t o = (size_t)t * per, l = len - o < per ? len - o : per;
-                th.emplace_back([=] { std::memcpy(dst + o, src + o, l); });
+                try {
+                    th.emplace_back([=] { std::memcpy(dst + o, src + o, l); });
+                } catch (...) {
+                    std::memcpy(dst + o, src + o, l);
+                }
             }
             std::memcpy(dst, src, per < len ? per : len);
             for (auto &x : th) x.join();
