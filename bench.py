@@ -211,7 +211,7 @@ def hbm_probe(torch, G, mib_per_stream: int = 512) -> dict:
 
 def bench_warmstart(args) -> dict:
     """Learned warm start, 640x480, 64 labels, TL lambda=10 tau=3: per seeded
-    evaluation instance (seeds 2000000...), plain HBP 5x10 (the judge) against
+    evaluation instance (seeds 2000000...), plain HBP 5x10 (the energy reference) against
     each seeded schedule, with the committed network and with zero seeds.
     value = time of the cheapest schedule that reaches plain HBP's energy
     within 0.1 % on every instance, divided by plain HBP's time."""
@@ -296,8 +296,8 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is designed, not built)"}))
         return 2
 
     import torch

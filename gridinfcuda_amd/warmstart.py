@@ -7,7 +7,7 @@ to 1/4 resolution (L input channels) to the four directional message volumes
 at 1/4 resolution (4L output channels).  Its prediction seeds a fine level of
 the BP pyramid (outbox-frame upsample, border outboxes kept at zero), which
 then runs a few red-black half-sweeps; plain hierarchical BP stays the
-fallback and the judge of the energy.
+fallback and the energy reference.
 
 Every layer runs on the library's own HIP kernels (ginf_conv3x3_* on MFMA);
 PyTorch only provides device memory and the stream.  Training targets are the
@@ -241,7 +241,7 @@ def gen_images(seeds, wl: Workload, threads: int = 16):
 
 class Pipeline:
     """Device-side instance preparation for one workload: costs from images,
-    plain HBP (the judge), targets and network inputs."""
+    plain HBP (the energy reference), targets and network inputs."""
 
     def __init__(self, wl: Workload):
         import torch
@@ -395,7 +395,7 @@ def _timed(torch, fn, reps=5):
 
 # (seed level, half-sweeps per level from the seeded level down to level 0);
 # seed level -1 is plain HBP from zero messages on all 5 levels (a baseline
-# with fewer iterations than the judge's 10 per level).
+# with fewer iterations than the reference's 10 per level).
 SCHEDULES = [(0, (4,)), (0, (6,)), (0, (8,)), (0, (10,)), (0, (14,)),
              (1, (2, 4)), (1, (4, 4)), (1, (4, 6)), (1, (6, 6)), (1, (6, 8)), (1, (10, 10)),
              (2, (4, 4, 6)), (2, (6, 6, 6)), (2, (6, 6, 8)), (2, (10, 10, 10)),
@@ -403,7 +403,7 @@ SCHEDULES = [(0, (4,)), (0, (6,)), (0, (8,)), (0, (10,)), (0, (14,)),
 
 
 def evaluate(wl: Workload, net: ConvNet | None, seeds, schedules=SCHEDULES, reps=5, log=print):
-    """Energy and GPU time of plain HBP (the judge) against seeded schedules,
+    """Energy and GPU time of plain HBP (the energy reference) against seeded schedules,
     per instance.  A schedule (level, iters) seeds `level` from the network
     (or from zeros when net is None) and runs iters[t] half-sweeps at level
     level-t ... 0."""

@@ -15,7 +15,7 @@ One pass over a chunk of instances:
      (ginf_batch_run: persistent work-list kernel, HIP graph replay);
   5. FP64 energies of the labellings (ginf_energy_f64).
 Plain hierarchical BP (5 levels x 10 half-sweeps, one plan per class) is the
-judge: an instance whose warm-started energy exceeds the plain energy by more
+energy reference: an instance whose warm-started energy exceeds the plain energy by more
 than 0.1 % falls back to the plain result, and its plain time is added to the
 warm-start time.  The same batch seeded with zeros (no network) is the
 ablation.
@@ -65,7 +65,7 @@ def fallback_flags(e_warm, e_plain, tol: float = FALLBACK_TOL):
 
 class BatchedWarmStart:
     """Device state for one chunk layout: per-instance costs and labels, per
-    class network inputs / predictions, the plain plans (judge) and a
+    class network inputs / predictions, the plain plans (energy reference) and a
     seeded batch of finest-level plans with K half-sweeps."""
 
     def __init__(self, classes, nets, chunk_classes, iters: int):
@@ -171,8 +171,8 @@ def load_nets(classes, paths=None):
 
 
 def evaluate_batched(n=256, base_seed=5000, chunk=64, iters=(16,), reps=3, nets=None, tune=True, log=print):
-    """Run the batched warm start over n instances in chunks and judge every
-    instance against plain HBP.  With tune, a chunk layout that has no cached
+    """Run the batched warm start over n instances in chunks and compare every
+    instance with plain HBP.  With tune, a chunk layout that has no cached
     tiles is tuned once (and cached) before it is timed.  Returns per-K
     summaries and per-instance records."""
     import torch
