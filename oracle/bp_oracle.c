@@ -4,8 +4,8 @@
  * A plain-C restatement of the CPU min-sum belief propagation of
  * GridInfCuda (src/bp.cc, with the energy of src/grid.cc:102-125 and the
  * Matrix layout of src/matrix.cc).  Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg load it, and only as the checker or as the
- * timed CPU baseline; the HIP library never calls it.
+ * bench.py's cpu_baseline leg load it, and only as the reference results are
+ * checked against or as the timed CPU baseline; the HIP library never calls it.
  *
  * How each function is pinned (tests/test_oracle_*.py):
  *   oracle_pyramid      brute-force child sums, ceil(w/2) dims (numpy)
