@@ -180,6 +180,136 @@ def bench_config1(args) -> dict:
     }
 
 
+def _stereo_table(O, L, lam, tau, dtype):
+    return O.table_trunc_linear(L, lam, lam * tau, dtype)
+
+
+def bench_icm(args) -> dict:
+    """ICM (src/icm.cc:5-62, ginf_icm) on the 1920x1080x128 stereo costs of
+    the headline workload (seed 2, TL lambda=8 tau=4, FP32), from the
+    decodeMaxPrior labelling, 10 iterations, device-resident, HIP events on
+    the launch stream (median of the timed runs).  Metric: node-label
+    evaluations/s (interior nodes x labels x iterations / time).  The
+    synchronous kernel reads every interior node's L costs once per iteration
+    (4 B per node-label) plus its label and writes the new one (8 B per node):
+    HBM-bound.  The asynchronous mode (the reference's in-place row-major
+    order) runs as W+H+2N anti-diagonal launches and is launch-bound.  CPU:
+    the oracle, one thread, on the same recipe at 960x540 (a quarter of the
+    nodes), whose labels the GPU reproduces."""
+    import torch
+    from gridinfcuda_amd import _lib as G
+    from oracle import oracle as O
+    W, H, L, N = 1920, 1080, 128, 10
+    s = torch.cuda.current_stream()
+
+    def gpu_run(Wc, Hc, mode, runs):
+        data = torch.from_numpy(G.gen_stereo(2, Wc, Hc, L, 64, L - 1, 40)["cost_f32"]).cuda()
+        table = torch.from_numpy(_stereo_table(O, L, 8, 4, np.float32)).cuda()
+        init = torch.zeros((Hc, Wc), dtype=torch.int32, device="cuda")
+        out = torch.zeros_like(init)
+        G.check(G.lib().ginf_max_prior(Wc, Hc, L, G.F32, ctypes.c_void_p(data.data_ptr()),
+                                       ctypes.c_void_p(init.data_ptr()), ctypes.c_void_p(s.cuda_stream)))
+        go = lambda: G.icm(Wc, Hc, L, G.F32, data.data_ptr(), table.data_ptr(), init.data_ptr(), N, mode,
+                           out.data_ptr(), s.cuda_stream)
+        for _ in range(max(1, args.warmup)):
+            go()
+        ts = []
+        for _ in range(runs):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(s)
+            go()
+            b.record(s)
+            b.synchronize()
+            ts.append(a.elapsed_time(b))
+        return float(np.median(ts)), data, init, out
+
+    runs = max(5, args.steps)
+    evals = (W - 2) * (H - 2) * L * N
+    sync_ms, data, init, _ = gpu_run(W, H, G.ICM_SYNC, runs)
+    async_ms, _, _, _ = gpu_run(W, H, G.ICM_ASYNC, max(3, runs // 2))
+    del data, init
+    algo = (W - 2) * (H - 2) * (4 * L + 8) * N
+    # CPU sample and its parity check
+    Ws, Hs = 960, 540
+    _, sd, sinit, sout = gpu_run(Ws, Hs, G.ICM_SYNC, 1)
+    cdata = G.gen_stereo(2, Ws, Hs, L, 64, L - 1, 40)["cost_f32"]
+    ctab = _stereo_table(O, L, 8, 4, np.float32)
+    cinit = O.max_prior(cdata)
+    t0 = time.perf_counter()
+    ref = O.icm(cdata, ctab, cinit, N, sync=True)
+    cpu_s = time.perf_counter() - t0
+    same = bool(np.array_equal(ref, sout.cpu().numpy())) and bool(np.array_equal(cinit, sinit.cpu().numpy()))
+    cpu_evals = (Ws - 2) * (Hs - 2) * L * N
+    return {
+        "metric": "Gnode-label ICM evaluations/s", "value": round(evals / sync_ms / 1e6, 3),
+        "unit": "Gnode-label evaluations/s", "n_gpus": 1, "steps": runs, "warmup": args.warmup,
+        "ms_per_step": round(sync_ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded stereo generator, seed 2)",
+        "config": {"workload": "ICM 1920x1080x128 TL lambda=8 tau=4, from decodeMaxPrior, 10 iterations",
+                   "sync_ms": round(sync_ms, 4), "async_ms": round(async_ms, 4),
+                   "async_launches": W + H + 2 * N, "labels_equal_oracle_at_960x540": same},
+        "roofline": {"bound": "hbm", "achieved": round(algo / sync_ms / 1e6, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(algo / sync_ms / 1e6 / 8000.0, 4), "traffic": None,
+                     "kernel": "synchronous ICM, 10 launches (4 B per node-label + 8 B per node and iteration)"},
+        "cpu_baseline": {"value": round(cpu_evals / cpu_s / 1e9, 5), "unit": "Gnode-label evaluations/s", "cores": 1,
+                         "kind": "port", "sample": "the same recipe at 960x540, 10 synchronous iterations, one run",
+                         "seconds": round(cpu_s, 3), "cpu_model": cpu_model()},
+    }
+
+
+def bench_alpha_beta(args) -> dict:
+    """Alpha-beta swap (src/graphcut.cc:194-279, ginf_alpha_beta) on int
+    stereo costs at the Tsukuba size: 384x288, 16 labels (seed 1, 8
+    rectangles, disparities 0..15, truncation 20), V = 10*min(|a-b|, 2), from
+    the decodeMaxPrior labelling, one cycle over all 120 label pairs;
+    device-resident, median of the timed runs.  Each pair is a maximum
+    preflow by parallel push-relabel (atomics, global relabels): latency- and
+    atomic-bound, so no roofline is claimed.  CPU: the oracle (each pair
+    solved by BFS augmenting paths), one thread, the full workload once; both
+    give the same labels (the sink-reachable set of a maximum flow is
+    canonical)."""
+    import torch
+    from gridinfcuda_amd import _lib as G
+    from oracle import oracle as O
+    W, H, L = 384, 288, 16
+    cost = G.gen_stereo(1, W, H, L, 8, 15, 20, want=("cost_i32",))["cost_i32"]
+    table = _stereo_table(O, L, 10, 2, np.int32)
+    s = torch.cuda.current_stream()
+    dd = torch.from_numpy(cost).cuda()
+    td = torch.from_numpy(table).cuda()
+    init = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    out = torch.zeros_like(init)
+    G.check(G.lib().ginf_max_prior(W, H, L, G.I32, ctypes.c_void_p(dd.data_ptr()), ctypes.c_void_p(init.data_ptr()),
+                                   ctypes.c_void_p(s.cuda_stream)))
+    go = lambda: G.alpha_beta(W, H, L, dd.data_ptr(), td.data_ptr(), init.data_ptr(), 1, out.data_ptr(), s.cuda_stream)
+    for _ in range(max(1, args.warmup)):
+        go()
+    runs = max(5, args.steps)
+    ts, moves = [], 0
+    for _ in range(runs):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        moves = go()
+        b.record(s)
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    gpu_ms = float(np.median(ts))
+    t0 = time.perf_counter()
+    ref, ref_moves = O.alpha_beta(cost, table, O.max_prior(cost), 1)
+    cpu_s = time.perf_counter() - t0
+    same = bool(np.array_equal(ref, out.cpu().numpy())) and ref_moves == moves
+    return {
+        "metric": "ms per alpha-beta cycle", "value": round(gpu_ms, 3), "unit": "ms", "n_gpus": 1, "steps": runs,
+        "warmup": args.warmup, "ms_per_step": round(gpu_ms, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded stereo generator, seed 1)",
+        "config": {"workload": "alpha-beta swap 384x288x16, V=10*min(|a-b|,2), one cycle over 120 label pairs",
+                   "moves_kept": moves, "labels_and_moves_equal_oracle": same},
+        "roofline": None,
+        "cpu_baseline": {"value": round(cpu_s * 1000.0, 1), "unit": "ms", "cores": 1, "kind": "port",
+                         "sample": "the full workload, one run", "cpu_model": cpu_model()},
+    }
+
+
 def hbm_probe(torch, G, mib_per_stream: int = 512) -> dict:
     """Best bandwidth of the library's streaming probe (ginf_stream_probe):
     5 read + 4 written float4 streams (the half-sweep's stream shape) and a
@@ -281,11 +411,13 @@ def main() -> int:
     ap.add_argument("--data32", action="store_true",
                     help="keep 32-bit data costs (no 16-bit storage even when the costs fit)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
-    ap.add_argument("--workload", default="hbp", choices=["hbp", "batched", "warmstart", "warmstart-batch", "config1"],
+    ap.add_argument("--workload", default="hbp", choices=["hbp", "batched", "warmstart", "warmstart-batch", "config1",
+                                                          "icm", "alpha-beta"],
                     help="hbp: 1920x1080x128 hierarchical BP (default); batched: 64 mixed MRFs; "
                          "warmstart: 640x480x64 learned warm start vs plain HBP; "
                          "warmstart-batch: 256-instance batched warm start with fallback; "
-                         "config1: 128x96x16 plain BP, GPU and one CPU core")
+                         "config1: 128x96x16 plain BP, GPU and one CPU core; "
+                         "icm: ICM on the default workload's costs; alpha-beta: 384x288x16 alpha-beta swap")
     ap.add_argument("--eval-instances", type=int, default=None,
                     help="warm-start workloads: instances evaluated (default 8 / 256)")
     ap.add_argument("--batch-iters", default="16", help="warmstart-batch: finest-level half-sweeps K (list)")
@@ -319,6 +451,12 @@ def main() -> int:
     if args.workload == "warmstart-batch":
         args.eval_instances = args.eval_instances or 256
         print(json.dumps(bench_warmstart_batch(args)))
+        return 0
+    if args.workload == "icm":
+        print(json.dumps(bench_icm(args)))
+        return 0
+    if args.workload == "alpha-beta":
+        print(json.dumps(bench_alpha_beta(args)))
         return 0
     W, H, L = args.width, args.height, args.labels
     gen = G.gen_stereo(2, W, H, L, 64, 127, 40)["cost_f32"]

@@ -15,8 +15,9 @@
 //     updates diagonal s-2t for every iteration t at once, which reproduces
 //     the sequential result exactly.
 // Costs stay in the reference layout [l][y][x]: a wave reads 64 adjacent x
-// of one label plane per load.  The L x L table V(a,b) at a + L*b is read
-// through the cache (column f of V is contiguous in a).
+// of one label plane per load.  The L x L table V(a,b) at a + L*b is staged
+// in LDS for synchronous ICM with L <= 128 and read through the cache
+// otherwise.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -42,19 +43,155 @@ __device__ __forceinline__ int icm_best(const T *__restrict__ data, const T *__r
     const T *vE = table + (int64_t)L * fE;
     const T *vN = table + (int64_t)L * fN;
     const T *vW = table + (int64_t)L * fW;
+    // Labels in groups of U: the group's cost loads are issued together (one
+    // cost plane apart each, independent of the running minimum), then the
+    // sums and the comparisons run in label order, so each wave waits for
+    // memory once per group instead of once per label.
+    constexpr int U = 16;
     T best = T(0);
     int arg = 0;
-    for (int l = 0; l < L; ++l) {
-        T c = data[(int64_t)l * plane + node];
-        c = c + vS[l];
-        c = c + vE[l];
-        c = c + vN[l];
-        c = c + vW[l];
-        if (l == 0 || c < best) { best = c; arg = l; }
+    for (int l0 = 0; l0 < L; l0 += U) {
+        T dv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) dv[u] = l0 + u < L ? data[(int64_t)(l0 + u) * plane + node] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int l = l0 + u;
+            if (l < L) {
+                T c = dv[u];
+                c = c + vS[l];
+                c = c + vE[l];
+                c = c + vN[l];
+                c = c + vW[l];
+                if (l == 0 || c < best) { best = c; arg = l; }
+            }
+        }
     }
     return arg;
 }
 
+// ---- table in LDS (synchronous ICM, L <= kLdsMaxLabels) -----------------
+// With the table in global memory every label of every node gathers V(l, f)
+// for four neighbour labels f that differ from lane to lane, 64 cache lines
+// per wave and load.  The kernels below stage the table in LDS once per
+// block: row f (a neighbour label) holds V(l, f) for l in [0, L) at
+// l ^ (f & 31) of a row of Lp = roundup(L, 32) entries, so lanes reading the
+// same l from different rows fall in different banks.  The sums and the
+// comparisons are those of icm_best, in the same order.
+constexpr int kLdsMaxLabels = 128;  // L * Lp * 4 B <= 64 KiB
+
+template <typename T>
+__device__ __forceinline__ void stage_table(T *lds, const T *__restrict__ table, int L, int Lp) {
+    for (int i = threadIdx.x; i < L * L; i += blockDim.x) {
+        const int l = i % L, f = i / L;  // table[l + L * f] = V(l, f)
+        lds[f * Lp + (l ^ (f & 31))] = table[i];
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ int icm_best_lds(const T *__restrict__ data, const T *tab, int Lp, int64_t plane,
+                                            int64_t node, int L, int fS, int fE, int fN, int fW) {
+    fS = clamp_label(fS, L);
+    fE = clamp_label(fE, L);
+    fN = clamp_label(fN, L);
+    fW = clamp_label(fW, L);
+    const T *rS = tab + fS * Lp, *rE = tab + fE * Lp, *rN = tab + fN * Lp, *rW = tab + fW * Lp;
+    const int sS = fS & 31, sE = fE & 31, sN = fN & 31, sW = fW & 31;
+    constexpr int U = 16;
+    T best = T(0);
+    int arg = 0;
+    for (int l0 = 0; l0 < L; l0 += U) {
+        T dv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) dv[u] = l0 + u < L ? data[(int64_t)(l0 + u) * plane + node] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int l = l0 + u;
+            if (l < L) {
+                T c = dv[u];
+                c = c + rS[l ^ sS];
+                c = c + rE[l ^ sE];
+                c = c + rN[l ^ sN];
+                c = c + rW[l ^ sW];
+                if (l == 0 || c < best) { best = c; arg = l; }
+            }
+        }
+    }
+    return arg;
+}
+
+// Synchronous iteration over the interior as one flat index (x fastest),
+// grid-stride with a grid sized to the machine, so each block stages the
+// table once.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_icm_sync_lds(const T *__restrict__ data, const T *__restrict__ table, int W,
+                                                       int H, int L, const int32_t *__restrict__ old,
+                                                       int32_t *__restrict__ out) {
+    extern __shared__ unsigned char smem_raw[];
+    T *tab = reinterpret_cast<T *>(smem_raw);
+    const int Lp = (L + 31) & ~31;
+    stage_table(tab, table, L, Lp);
+    const int wi = W - 2;
+    const int64_t nin = (int64_t)wi * (H - 2);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nin; i += (int64_t)gridDim.x * blockDim.x) {
+        const int y = 1 + (int)(i / wi), x = 1 + (int)(i - (int64_t)(y - 1) * wi);
+        const int64_t n = (int64_t)y * W + x;
+        out[n] = icm_best_lds(data, tab, Lp, (int64_t)W * H, n, L, old[n + W], old[n + 1], old[n - W], old[n - 1]);
+    }
+}
+
+// Launch s of asynchronous ICM: iteration t = t0 + blockIdx.y updates
+// anti-diagonal j = s - 2t (x + y = 2 + j) of the interior, in place, with 16
+// lanes per node: lane q takes labels q, q+16,
+// ... (so the 16 lanes of a node read 16 consecutive entries of each table
+// column), forms each label's cost in the order of icm_best, keeps its
+// lowest-index minimum, and the lanes combine: the smaller cost wins, ties go
+// to the lower label.  That is the sequential scan's result for finite costs,
+// and a step's few nodes now wait for L/16 table reads each instead of L.
+constexpr int kAsyncLanes = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_icm_async_lanes(const T *__restrict__ data, const T *__restrict__ table,
+                                                         int W, int H, int L, int32_t *__restrict__ lab, int s, int t0,
+                                                         int nd) {
+    const int t = t0 + blockIdx.y;
+    const int j = s - 2 * t;
+    if (j < 0 || j >= nd) return;  // uniform per block
+    const int k = 2 + j;
+    const int xs = max(1, k - (H - 2)), xe = min(W - 2, k - 1);
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = xs + gid / kAsyncLanes, q = gid % kAsyncLanes;
+    const bool active = x <= xe;  // inactive lanes still join the shuffles
+    const int y = k - x;
+    const int64_t n = (int64_t)y * W + x;
+    const int64_t plane = (int64_t)W * H;
+    T best = T(0);
+    int arg = -1;
+    if (active) {
+        const int fS = clamp_label(lab[n + W], L), fE = clamp_label(lab[n + 1], L);
+        const int fN = clamp_label(lab[n - W], L), fW = clamp_label(lab[n - 1], L);
+        const T *vS = table + (int64_t)L * fS, *vE = table + (int64_t)L * fE;
+        const T *vN = table + (int64_t)L * fN, *vW = table + (int64_t)L * fW;
+        for (int l = q; l < L; l += kAsyncLanes) {
+            T c = data[(int64_t)l * plane + n];
+            c = c + vS[l];
+            c = c + vE[l];
+            c = c + vN[l];
+            c = c + vW[l];
+            if (arg < 0 || c < best) { best = c; arg = l; }
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < kAsyncLanes; m <<= 1) {
+        const T ob = __shfl_xor(best, m, kAsyncLanes);
+        const int oa = __shfl_xor(arg, m, kAsyncLanes);
+        if (oa >= 0 && (arg < 0 || ob < best || (!(best < ob) && oa < arg))) { best = ob; arg = oa; }
+    }
+    if (active && q == 0) lab[n] = arg;
+}
+
+// ---- table in global memory (any L) ---------------------------------------
 template <typename T>
 __global__ void k_icm_sync(const T *__restrict__ data, const T *__restrict__ table, int W, int H, int L,
                            const int32_t *__restrict__ old, int32_t *__restrict__ out) {
@@ -63,23 +200,6 @@ __global__ void k_icm_sync(const T *__restrict__ data, const T *__restrict__ tab
     if (x > W - 2 || y > H - 2) return;
     const int64_t n = (int64_t)y * W + x;
     out[n] = icm_best(data, table, (int64_t)W * H, n, L, old[n + W], old[n + 1], old[n - W], old[n - 1]);
-}
-
-// Launch s: iteration t = t0 + blockIdx.y updates anti-diagonal j = s - 2t
-// (x + y = 2 + j) of the interior, in place.
-template <typename T>
-__global__ void k_icm_async_step(const T *__restrict__ data, const T *__restrict__ table, int W, int H, int L,
-                                 int32_t *__restrict__ lab, int s, int t0, int nd) {
-    const int t = t0 + blockIdx.y;
-    const int j = s - 2 * t;
-    if (j < 0 || j >= nd) return;
-    const int k = 2 + j;
-    const int xs = max(1, k - (H - 2)), xe = min(W - 2, k - 1);
-    const int x = xs + blockIdx.x * blockDim.x + threadIdx.x;
-    if (x > xe) return;
-    const int y = k - x;
-    const int64_t n = (int64_t)y * W + x;
-    lab[n] = icm_best(data, table, (int64_t)W * H, n, L, lab[n + W], lab[n + 1], lab[n - W], lab[n - 1]);
 }
 
 template <typename T>
@@ -103,15 +223,27 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
     hipError_t e = hipSuccess;
     if (result != initial) e = hipMemcpyAsync(result, initial, nl, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess || iters == 0 || W < 3 || H < 3) return e;
+    const bool lds = L <= kLdsMaxLabels;
+    const size_t lds_bytes = (size_t)L * ((L + 31) & ~31) * sizeof(T);
     if (mode == GINF_ICM_SYNC) {
         int32_t *tmp = nullptr;
         if ((e = hipMallocAsync((void **)&tmp, nl, st)) != hipSuccess) return e;
         e = hipMemcpyAsync(tmp, initial, nl, hipMemcpyDeviceToDevice, st);
         // The last iteration writes `result`.
         int32_t *bufs[2] = {(iters & 1) ? tmp : result, (iters & 1) ? result : tmp};
+        int dev = 0, cus = 0;
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        const int64_t nin = (int64_t)(W - 2) * (H - 2);
+        const unsigned lds_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nin + 1023) / 1024, 2 * (int64_t)cus));
         const dim3 grid((unsigned)((W - 2 + 255) / 256), (unsigned)(H - 2)), block(256);
         for (int t = 0; t < iters && e == hipSuccess; ++t) {
-            hipLaunchKernelGGL(k_icm_sync<T>, grid, block, 0, st, data, table, W, H, L, bufs[t & 1], bufs[(t + 1) & 1]);
+            if (lds)
+                hipLaunchKernelGGL(k_icm_sync_lds<T>, dim3(lds_grid), dim3(1024), lds_bytes, st, data, table, W, H, L,
+                                   bufs[t & 1], bufs[(t + 1) & 1]);
+            else
+                hipLaunchKernelGGL(k_icm_sync<T>, grid, block, 0, st, data, table, W, H, L, bufs[t & 1],
+                                   bufs[(t + 1) & 1]);
             e = hipGetLastError();
         }
         const hipError_t f = hipFreeAsync(tmp, st);
@@ -125,8 +257,11 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
         const int t_lo = s < nd ? 0 : (s - nd) / 2 + 1;
         const int t_hi = min(iters - 1, s / 2);
         if (t_lo > t_hi) continue;
-        const dim3 grid((unsigned)((longest + 255) / 256), (unsigned)(t_hi - t_lo + 1)), block(256);
-        hipLaunchKernelGGL(k_icm_async_step<T>, grid, block, 0, st, data, table, W, H, L, result, s, t_lo, nd);
+        // (the table stays in global memory here: a launch has too few nodes
+        // to repay staging it in every block; measured 2x slower with LDS)
+        const int64_t lanes = (int64_t)longest * kAsyncLanes;
+        const dim3 grid((unsigned)((lanes + 255) / 256), (unsigned)(t_hi - t_lo + 1)), block(256);
+        hipLaunchKernelGGL(k_icm_async_lanes<T>, grid, block, 0, st, data, table, W, H, L, result, s, t_lo, nd);
         e = hipGetLastError();
     }
     return e;
