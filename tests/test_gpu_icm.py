@@ -27,7 +27,12 @@ def case(seed, L, H, W, dtype, kind):
 @pytest.mark.parametrize("mode", [G.ICM_SYNC, G.ICM_ASYNC])
 @pytest.mark.parametrize("L,H,W,iters,kind", [(16, 37, 53, 5, "tl"), (7, 3, 40, 3, "explicit"), (5, 30, 3, 4, "potts"),
                                               (32, 129, 300, 2, "explicit"), (3, 2, 9, 3, "tl"), (9, 17, 11, 0, "tl"),
-                                              (64, 90, 70, 6, "potts"), (4, 400, 20, 7, "explicit")])
+                                              (64, 90, 70, 6, "potts"), (4, 400, 20, 7, "explicit"),
+                                              # table in LDS: full 64 KiB, padded rows; then in global memory
+                                              (128, 40, 50, 3, "explicit"), (100, 33, 45, 4, "tl"),
+                                              (150, 20, 30, 2, "potts"),
+                                              # more interior nodes than one pass of the synchronous grid
+                                              (8, 700, 800, 3, "tl")])
 def test_icm_matches_oracle(oracle_mod, dtype, mode, L, H, W, iters, kind):
     torch = torch_cuda()
     say(f"icm mode={mode} {np.dtype(dtype).name} L={L} H={H} W={W} iters={iters} {kind}")
