@@ -310,6 +310,68 @@ def bench_alpha_beta(args) -> dict:
     }
 
 
+def bench_stereo(args) -> dict:
+    """The reference's stereo example (examples/stereo/stereo.cc:25-51) on the
+    Tsukuba pair it ships: 384x288, 16 labels, Grid<int> costs
+    min(|L - R(x-k)|, 20), TL (10, 20), decodeBp with 30 half-sweeps
+    (reference mean normalisation, integer arithmetic).  GPU: device-resident
+    plan runs (median of >= 100) and the synchronous host entry the C++
+    example's decodeBp goes through (median of 20, copies included); CPU: the
+    oracle on one thread, median of 5 runs of the full workload."""
+    import torch
+    from gridinfcuda_amd import _lib as G
+    from gridinfcuda_amd import stereo as S
+    from oracle import oracle as O
+    gold = os.path.join(ROOT, "tests", "golden")
+    left, right = S.load_gray(os.path.join(gold, "tsukuba_left.png")), S.load_gray(os.path.join(gold, "tsukuba_right.png"))
+    costs = S.example_costs(left, right)
+    table = S.smoothness_table()
+    L, H, W = costs.shape
+    iters = S.ITERS
+    plan = G.Plan(G.make_desc(W, H, L, G.I32, G.SM_TRUNC_LINEAR, 1, iters, G.NORM_MEAN), table)
+    dd = torch.from_numpy(np.ascontiguousarray(costs)).cuda()
+    lab = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    s = torch.cuda.current_stream()
+    for _ in range(max(1, args.warmup)):
+        plan.run(dd.data_ptr(), lab.data_ptr(), s.cuda_stream)
+    runs = max(100, args.steps)
+    ts = []
+    for _ in range(runs):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        plan.run(dd.data_ptr(), lab.data_ptr(), s.cuda_stream)
+        b.record(s)
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    gpu_ms = float(np.median(ts))
+    host = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        hl = G.hbp_host(costs, table, G.SM_TRUNC_LINEAR, 1, iters)
+        host.append((time.perf_counter() - t0) * 1000.0)
+    cts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        ref = O.hbp(costs, table, G.SM_TRUNC_LINEAR, 1, iters)
+        cts.append((time.perf_counter() - t0) * 1000.0)
+    cpu_ms = float(np.median(cts))
+    U = plan.updates_per_run()
+    same = bool(np.array_equal(ref, lab.cpu().numpy())) and bool(np.array_equal(ref, hl))
+    return {
+        "metric": "Gnode-label msg updates/s", "value": round(U / gpu_ms / 1e6, 3), "unit": "Gnode-label msg updates/s",
+        "n_gpus": 1, "steps": runs, "warmup": args.warmup, "ms_per_step": round(gpu_ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "Tsukuba stereo pair shipped with the reference example",
+        "config": {"workload": "stereo example 384x288x16, Grid<int>, TL (10, 20), decodeBp 30 half-sweeps",
+                   "node_label_updates_per_step": U, "host_entry_ms": round(float(np.median(host)), 4),
+                   "labels_equal_oracle": same},
+        "roofline": None,
+        "cpu_baseline": {"value": round(U / cpu_ms / 1e6, 5), "unit": "Gnode-label msg updates/s", "cores": 1,
+                         "kind": "port", "sample": "the full workload, median of 5 runs", "ms_per_step": round(cpu_ms, 3),
+                         "cpu_model": cpu_model()},
+    }
+
+
 def hbm_probe(torch, G, mib_per_stream: int = 512) -> dict:
     """Best bandwidth of the library's streaming probe (ginf_stream_probe):
     5 read + 4 written float4 streams (the half-sweep's stream shape) and a
@@ -412,12 +474,13 @@ def main() -> int:
                     help="keep 32-bit data costs (no 16-bit storage even when the costs fit)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
     ap.add_argument("--workload", default="hbp", choices=["hbp", "batched", "warmstart", "warmstart-batch", "config1",
-                                                          "icm", "alpha-beta"],
+                                                          "icm", "alpha-beta", "stereo"],
                     help="hbp: 1920x1080x128 hierarchical BP (default); batched: 64 mixed MRFs; "
                          "warmstart: 640x480x64 learned warm start vs plain HBP; "
                          "warmstart-batch: 256-instance batched warm start with fallback; "
                          "config1: 128x96x16 plain BP, GPU and one CPU core; "
-                         "icm: ICM on the default workload's costs; alpha-beta: 384x288x16 alpha-beta swap")
+                         "icm: ICM on the default workload's costs; alpha-beta: 384x288x16 alpha-beta swap; "
+                         "stereo: the reference example on the Tsukuba pair")
     ap.add_argument("--eval-instances", type=int, default=None,
                     help="warm-start workloads: instances evaluated (default 8 / 256)")
     ap.add_argument("--batch-iters", default="16", help="warmstart-batch: finest-level half-sweeps K (list)")
@@ -457,6 +520,9 @@ def main() -> int:
         return 0
     if args.workload == "alpha-beta":
         print(json.dumps(bench_alpha_beta(args)))
+        return 0
+    if args.workload == "stereo":
+        print(json.dumps(bench_stereo(args)))
         return 0
     W, H, L = args.width, args.height, args.labels
     gen = G.gen_stereo(2, W, H, L, 64, 127, 40)["cost_f32"]
