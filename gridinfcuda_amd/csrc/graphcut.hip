@@ -173,7 +173,8 @@ hipError_t read_counters(Ctx &c, Flow &g) {
 // Launches between host checks of the convergence counters: BFS steps of
 // the global relabel, and push/relabel rounds of the maximum preflow.  The
 // in-place BFS converges in few steps, so its batch is short (384x288x16,
-// one cycle: 64.0 ms with batches 4/8, 82.8-87.2 with 8/8, 92.7 with 2/4).
+// one cycle, with a counter clear before every step: 64.0 ms with batches
+// 4/8, 82.8-87.2 with 8/8, 92.7 with 2/4).
 #ifndef GINF_AB_BFS_BATCH
 #define GINF_AB_BFS_BATCH 4
 #endif
@@ -186,9 +187,13 @@ hipError_t global_relabel(Ctx &c, Flow &g) {
     hipError_t e = hipSuccess;
     // Seeds: distance 1 with sink residue, else kInf (partition nodes only).
     const int64_t N = (int64_t)g.W * g.H;
+    // Only the last step of a batch is counted (a step that changes nothing
+    // is the fixed point), so the counter is cleared just before it.
     for (int64_t guard = 0; guard <= N + 1; guard += GINF_AB_BFS_BATCH) {
-        hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, c.st, g.counter, 2);
-        for (int k = 0; k < GINF_AB_BFS_BATCH; ++k) hipLaunchKernelGGL(k_ab_bfs, dim3(c.blocks), dim3(256), 0, c.st, g);
+        for (int k = 0; k < GINF_AB_BFS_BATCH; ++k) {
+            if (k == GINF_AB_BFS_BATCH - 1) hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, c.st, g.counter, 2);
+            hipLaunchKernelGGL(k_ab_bfs, dim3(c.blocks), dim3(256), 0, c.st, g);
+        }
         if ((e = hipGetLastError()) != hipSuccess || (e = read_counters(c, g)) != hipSuccess) return e;
         if (c.host[1] == 0) return hipSuccess;
     }
@@ -208,7 +213,8 @@ hipError_t max_preflow(Ctx &c, Flow &g, int64_t max_rounds) {
         hipLaunchKernelGGL(k_ab_reset_heights, dim3(c.blocks), dim3(256), 0, c.st, g);
         if ((e = global_relabel(c, g)) != hipSuccess) return e;
         for (int k = 0; k < kRounds; ++k) {
-            hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, c.st, g.counter, 2);
+            // the host reads the last round's count of active nodes only
+            if (k == kRounds - 1) hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, c.st, g.counter, 2);
             hipLaunchKernelGGL(k_ab_push, dim3(c.blocks), dim3(256), 0, c.st, g);
             hipLaunchKernelGGL(k_ab_relabel, dim3(c.blocks), dim3(256), 0, c.st, g);
             std::swap(g.h, g.h2);
