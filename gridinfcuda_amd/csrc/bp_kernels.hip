@@ -32,7 +32,7 @@ __global__ __launch_bounds__(256) void k_relayout(const T *__restrict__ src, int
 // Relayout fused with the first pyramid level.  A block takes fine rows
 // 2Y and 2Y+1 and TW consecutive columns of the reference-layout costs
 // [l][y][x], stages them in LDS (coalesced reads along x), then writes the
-// level-0 state (both colours; each label plane's slots are contiguous) and,
+// level-0 state (both colours; a plane's slots are contiguous within a slot tile) and,
 // when `coarse` is set, the level-1 costs of coarse row Y: the children
 // (2X,2Y), (2X+1,2Y), (2X,2Y+1), (2X+1,2Y+1) added in that order from 0
 // (src/bp.cc:100-117).  Padding labels (L <= l < C*G) get cost 0.
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void k_relayout_pyr(const T *__restrict__ src,
         }
     }
     __syncthreads();
-    // level 0: per (row, colour, plane i): HS slots x G lanes, contiguous in memory
+    // level 0: per (row, colour, plane i): HS slots x G lanes, contiguous within a slot tile
     constexpr int HS = TW / 2;
     {
         const int per = HS << gs;          // elements per (row, colour, plane)
