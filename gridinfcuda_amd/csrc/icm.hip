@@ -141,6 +141,96 @@ __global__ __launch_bounds__(1024) void k_icm_sync_lds(const T *__restrict__ dat
     }
 }
 
+// The same iteration with four nodes x0..x0+3 (x0 % 4 == 0) per thread, for
+// W % 4 == 0, L % 4 == 0 and 16-byte aligned costs and label buffers: one 16-byte load
+// brings a label's costs of the four nodes (a wave reads 1 KiB of one plane
+// per load instead of 256 B), and the labels are read and written as int4.
+// A quad may hold the border columns x = 0 and x = W-1; those nodes keep
+// their old label, which both label buffers hold unchanged.  Per node the
+// sums and the comparisons are those of icm_best.
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<int> { using type = int4; };
+
+template <typename T>
+__global__ __launch_bounds__(768) void k_icm_sync_lds_x4(const T *__restrict__ data, const T *__restrict__ table,
+                                                          int W, int H, int L, const int32_t *__restrict__ old,
+                                                          int32_t *__restrict__ out) {
+    using V4 = typename Vec4<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    T *tab = reinterpret_cast<T *>(smem_raw);
+    const int Lp = (L + 31) & ~31;
+    stage_table(tab, table, L, Lp);
+    const int qw = W >> 2;
+    const int64_t nq = (int64_t)qw * (H - 2), plane = (int64_t)W * H;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
+        const int y = 1 + (int)(i / qw), x0 = 4 * (int)(i - (int64_t)(y - 1) * qw);
+        const int64_t n0 = (int64_t)y * W + x0;
+        const int4 fc = *reinterpret_cast<const int4 *>(old + n0);
+        const int4 fs = *reinterpret_cast<const int4 *>(old + n0 + W);
+        const int4 fn = *reinterpret_cast<const int4 *>(old + n0 - W);
+        const int fw0 = x0 > 0 ? old[n0 - 1] : 0;      // only read for an interior node
+        const int fe3 = x0 + 4 < W ? old[n0 + 4] : 0;  // only read for an interior node
+        const int cur[4] = {fc.x, fc.y, fc.z, fc.w};
+        const int nS[4] = {fs.x, fs.y, fs.z, fs.w}, nN[4] = {fn.x, fn.y, fn.z, fn.w};
+        const int nE[4] = {fc.y, fc.z, fc.w, fe3}, nW[4] = {fw0, fc.x, fc.y, fc.z};
+        int oS[4], oE[4], oN[4], oW[4], sS[4], sE[4], sN[4], sW[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int a = clamp_label(nS[k], L), b = clamp_label(nE[k], L);
+            const int c = clamp_label(nN[k], L), d = clamp_label(nW[k], L);
+            oS[k] = a * Lp; sS[k] = a & 31;
+            oE[k] = b * Lp; sE[k] = b & 31;
+            oN[k] = c * Lp; sN[k] = c & 31;
+            oW[k] = d * Lp; sW[k] = d & 31;
+        }
+        T best[4] = {T(0), T(0), T(0), T(0)};
+        int arg[4] = {0, 0, 0, 0};
+        // Labels in groups of U (L % U == 0), software-pipelined: the next
+        // group's cost loads are issued before the current group's sums, so
+        // every wave keeps a group of loads in flight while it computes (the
+        // last group re-reads plane L-1 instead of branching).
+        constexpr int U = 4;
+        V4 dv[U], nx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) dv[u] = *reinterpret_cast<const V4 *>(data + (int64_t)u * plane + n0);
+        for (int l0 = 0; l0 < L; l0 += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                nx[u] = *reinterpret_cast<const V4 *>(data + (int64_t)min(l0 + U + u, L - 1) * plane + n0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int l = l0 + u;
+                const T d4[4] = {dv[u].x, dv[u].y, dv[u].z, dv[u].w};
+                // the label's 16 table reads first, then the sums: the reads
+                // are issued back to back instead of one LDS wait per add
+                T v[4][4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    v[k][0] = tab[oS[k] + (l ^ sS[k])];
+                    v[k][1] = tab[oE[k] + (l ^ sE[k])];
+                    v[k][2] = tab[oN[k] + (l ^ sN[k])];
+                    v[k][3] = tab[oW[k] + (l ^ sW[k])];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    T c = d4[k];
+                    c = c + v[k][0];
+                    c = c + v[k][1];
+                    c = c + v[k][2];
+                    c = c + v[k][3];
+                    if (l == 0 || c < best[k]) { best[k] = c; arg[k] = l; }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) dv[u] = nx[u];
+        }
+        const bool xl = x0 == 0, xr = x0 + 4 == W;  // border columns keep their label
+        *reinterpret_cast<int4 *>(out + n0) =
+            int4{xl ? cur[0] : arg[0], arg[1], arg[2], xr ? cur[3] : arg[3]};
+    }
+}
+
 // Launch s of asynchronous ICM: iteration t = t0 + blockIdx.y updates
 // anti-diagonal j = s - 2t (x + y = 2 + j) of the interior, in place, with 16
 // lanes per node: lane q takes labels q, q+16,
@@ -235,10 +325,20 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
         if (e == hipSuccess) e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         const int64_t nin = (int64_t)(W - 2) * (H - 2);
-        const unsigned lds_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nin + 1023) / 1024, 2 * (int64_t)cus));
+        // Four nodes per thread where the quads are whole and 16-byte aligned.
+        const bool x4 = lds && W % 4 == 0 && L % 4 == 0 && ((uintptr_t)data & 15) == 0 && ((uintptr_t)result & 15) == 0 &&
+                        ((uintptr_t)tmp & 15) == 0;
+        // (x4: 768-thread blocks, two per CU with 64 KiB of LDS each: six waves
+        // per SIMD, which its 80 VGPRs allow)
+        const int64_t items = x4 ? (int64_t)(W / 4) * (H - 2) : nin;
+        const int bs = x4 ? 768 : 1024;
+        const unsigned lds_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + bs - 1) / bs, 2 * (int64_t)cus));
         const dim3 grid((unsigned)((W - 2 + 255) / 256), (unsigned)(H - 2)), block(256);
         for (int t = 0; t < iters && e == hipSuccess; ++t) {
-            if (lds)
+            if (x4)
+                hipLaunchKernelGGL(k_icm_sync_lds_x4<T>, dim3(lds_grid), dim3(bs), lds_bytes, st, data, table, W, H,
+                                   L, bufs[t & 1], bufs[(t + 1) & 1]);
+            else if (lds)
                 hipLaunchKernelGGL(k_icm_sync_lds<T>, dim3(lds_grid), dim3(1024), lds_bytes, st, data, table, W, H, L,
                                    bufs[t & 1], bufs[(t + 1) & 1]);
             else

@@ -32,7 +32,11 @@ def case(seed, L, H, W, dtype, kind):
                                               (128, 40, 50, 3, "explicit"), (100, 33, 45, 4, "tl"),
                                               (150, 20, 30, 2, "potts"),
                                               # more interior nodes than one pass of the synchronous grid
-                                              (8, 700, 800, 3, "tl")])
+                                              (8, 700, 800, 3, "tl"),
+                                              # W % 4 == 0: four nodes per thread (one quad holding both
+                                              # border columns; a full 128-label table in LDS)
+                                              (6, 9, 4, 3, "tl"), (128, 24, 64, 3, "explicit"),
+                                              (33, 50, 28, 4, "potts")])
 def test_icm_matches_oracle(oracle_mod, dtype, mode, L, H, W, iters, kind):
     torch = torch_cuda()
     say(f"icm mode={mode} {np.dtype(dtype).name} L={L} H={H} W={W} iters={iters} {kind}")
@@ -50,6 +54,24 @@ def test_icm_matches_oracle(oracle_mod, dtype, mode, L, H, W, iters, kind):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(idev.cpu().numpy(), want)
     np.testing.assert_array_equal(G.icm_host(data, table, init, iters, mode), want)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float32])
+def test_icm_sync_unaligned_buffers(oracle_mod, dtype):
+    """W % 4 == 0 but a result buffer 4 bytes past a 16-byte boundary: the
+    one-node-per-thread kernel runs, with the same labels."""
+    torch = torch_cuda()
+    L, H, W, iters = 24, 31, 44, 3
+    say(f"icm sync unaligned {np.dtype(dtype).name} L={L} H={H} W={W}")
+    data, table, init = case(5, L, H, W, dtype, "tl")
+    want = oracle_mod.icm(data, table, init, iters, sync=True)
+    dd, td, idev = (torch.from_numpy(a).cuda() for a in (data, table, init))
+    base = torch.full((H * W + 4,), -1, dtype=torch.int32, device="cuda")
+    out = base[1:1 + H * W]
+    G.icm(W, H, L, G.dtype_code(data.dtype), dd.data_ptr(), td.data_ptr(), idev.data_ptr(), iters, G.ICM_SYNC,
+          out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy().reshape(H, W), want)
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.float32])
