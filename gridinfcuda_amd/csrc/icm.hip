@@ -142,15 +142,42 @@ __global__ __launch_bounds__(1024) void k_icm_sync_lds(const T *__restrict__ dat
 }
 
 // The same iteration with four nodes x0..x0+3 (x0 % 4 == 0) per thread, for
-// W % 4 == 0, L % 4 == 0 and 16-byte aligned costs and label buffers: one 16-byte load
-// brings a label's costs of the four nodes (a wave reads 1 KiB of one plane
-// per load instead of 256 B), and the labels are read and written as int4.
-// A quad may hold the border columns x = 0 and x = W-1; those nodes keep
-// their old label, which both label buffers hold unchanged.  Per node the
-// sums and the comparisons are those of icm_best.
+// W % 4 == 0, L % 4 == 0 and 16-byte aligned costs and label buffers: one
+// 16-byte load brings a label's costs of the four nodes (a wave reads 1 KiB
+// of one plane per load instead of 256 B), and the labels are read and
+// written as int4.  A quad may hold the border columns x = 0 and x = W-1;
+// those nodes keep their old label, which both label buffers hold unchanged.
+//
+// The table is staged in label quads: row f holds V(l, f) at
+// l ^ ((f & 7) << 2) of a row of Lr = max(32, 2^ceil(log2 L)) entries, so the
+// four labels l0..l0+3 (l0 % 4 == 0) of a row are one aligned 16-byte LDS
+// read, rows with different f & 7 start their quads in different banks, and
+// with the power-of-two stride a quad's byte address is (row base | swizzle)
+// XOR 4*l0: one VALU operation per neighbour and four labels.  Per node the
+// sums and the comparisons are those of icm_best, in the same order.
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { using type = float4; };
 template <> struct Vec4<int> { using type = int4; };
+
+__host__ __device__ inline int quad_row_stride(int L) {
+    int r = 32;
+    while (r < L) r <<= 1;
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void stage_table_quads(T *lds, const T *__restrict__ table, int L, int Lr) {
+    for (int i = threadIdx.x; i < L * L; i += blockDim.x) {
+        const int l = i % L, f = i / L;  // table[l + L * f] = V(l, f)
+        lds[f * Lr + (l ^ ((f & 7) << 2))] = table[i];
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_of(const typename Vec4<T>::type &v, int k) {
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
 
 template <typename T>
 __global__ __launch_bounds__(768) void k_icm_sync_lds_x4(const T *__restrict__ data, const T *__restrict__ table,
@@ -158,9 +185,8 @@ __global__ __launch_bounds__(768) void k_icm_sync_lds_x4(const T *__restrict__ d
                                                           int32_t *__restrict__ out) {
     using V4 = typename Vec4<T>::type;
     extern __shared__ unsigned char smem_raw[];
-    T *tab = reinterpret_cast<T *>(smem_raw);
-    const int Lp = (L + 31) & ~31;
-    stage_table(tab, table, L, Lp);
+    const int Lr = quad_row_stride(L);
+    stage_table_quads(reinterpret_cast<T *>(smem_raw), table, L, Lr);
     const int qw = W >> 2;
     const int64_t nq = (int64_t)qw * (H - 2), plane = (int64_t)W * H;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
@@ -172,58 +198,49 @@ __global__ __launch_bounds__(768) void k_icm_sync_lds_x4(const T *__restrict__ d
         const int fw0 = x0 > 0 ? old[n0 - 1] : 0;      // only read for an interior node
         const int fe3 = x0 + 4 < W ? old[n0 + 4] : 0;  // only read for an interior node
         const int cur[4] = {fc.x, fc.y, fc.z, fc.w};
-        const int nS[4] = {fs.x, fs.y, fs.z, fs.w}, nN[4] = {fn.x, fn.y, fn.z, fn.w};
-        const int nE[4] = {fc.y, fc.z, fc.w, fe3}, nW[4] = {fw0, fc.x, fc.y, fc.z};
-        int oS[4], oE[4], oN[4], oW[4], sS[4], sE[4], sN[4], sW[4];
+        // S, E, N, W neighbour labels of node k
+        const int nb[4][4] = {{fs.x, fc.y, fn.x, fw0}, {fs.y, fc.z, fn.y, fc.x},
+                              {fs.z, fc.w, fn.z, fc.y}, {fs.w, fe3, fn.w, fc.z}};
+        int P[4][4];  // byte address of row f's quad 0, swizzle included
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int a = clamp_label(nS[k], L), b = clamp_label(nE[k], L);
-            const int c = clamp_label(nN[k], L), d = clamp_label(nW[k], L);
-            oS[k] = a * Lp; sS[k] = a & 31;
-            oE[k] = b * Lp; sE[k] = b & 31;
-            oN[k] = c * Lp; sN[k] = c & 31;
-            oW[k] = d * Lp; sW[k] = d & 31;
-        }
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const int f = clamp_label(nb[k][d], L);
+                P[k][d] = (f * Lr * (int)sizeof(T)) | ((f & 7) << 4);
+            }
         T best[4] = {T(0), T(0), T(0), T(0)};
         int arg[4] = {0, 0, 0, 0};
-        // Labels in groups of U (L % U == 0), software-pipelined: the next
-        // group's cost loads are issued before the current group's sums, so
-        // every wave keeps a group of loads in flight while it computes (the
-        // last group re-reads plane L-1 instead of branching).
-        constexpr int U = 4;
-        V4 dv[U], nx[U];
+        // Labels in quads, software-pipelined: the next quad's cost loads are
+        // issued before the current quad's sums, so every wave keeps loads in
+        // flight while it computes (the last quad re-reads plane L-1 instead
+        // of branching).
+        V4 dv[4], nx[4];
 #pragma unroll
-        for (int u = 0; u < U; ++u) dv[u] = *reinterpret_cast<const V4 *>(data + (int64_t)u * plane + n0);
-        for (int l0 = 0; l0 < L; l0 += U) {
+        for (int u = 0; u < 4; ++u) dv[u] = *reinterpret_cast<const V4 *>(data + (int64_t)u * plane + n0);
+        for (int l0 = 0; l0 < L; l0 += 4) {
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                nx[u] = *reinterpret_cast<const V4 *>(data + (int64_t)min(l0 + U + u, L - 1) * plane + n0);
+            for (int u = 0; u < 4; ++u)
+                nx[u] = *reinterpret_cast<const V4 *>(data + (int64_t)min(l0 + 4 + u, L - 1) * plane + n0);
+            const int lb = l0 * (int)sizeof(T);
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int l = l0 + u;
-                const T d4[4] = {dv[u].x, dv[u].y, dv[u].z, dv[u].w};
-                // the label's 16 table reads first, then the sums: the reads
-                // are issued back to back instead of one LDS wait per add
-                T v[4][4];
+            for (int k = 0; k < 4; ++k) {
+                V4 t[4];  // V(l0..l0+3, f) for the S, E, N, W neighbours of node k
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    v[k][0] = tab[oS[k] + (l ^ sS[k])];
-                    v[k][1] = tab[oE[k] + (l ^ sE[k])];
-                    v[k][2] = tab[oN[k] + (l ^ sN[k])];
-                    v[k][3] = tab[oW[k] + (l ^ sW[k])];
-                }
+                for (int d = 0; d < 4; ++d) t[d] = *reinterpret_cast<const V4 *>(smem_raw + (P[k][d] ^ lb));
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    T c = d4[k];
-                    c = c + v[k][0];
-                    c = c + v[k][1];
-                    c = c + v[k][2];
-                    c = c + v[k][3];
+                for (int u = 0; u < 4; ++u) {
+                    const int l = l0 + u;
+                    T c = lane_of<T>(dv[u], k);
+                    c = c + lane_of<T>(t[0], u);
+                    c = c + lane_of<T>(t[1], u);
+                    c = c + lane_of<T>(t[2], u);
+                    c = c + lane_of<T>(t[3], u);
                     if (l == 0 || c < best[k]) { best[k] = c; arg[k] = l; }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) dv[u] = nx[u];
+            for (int u = 0; u < 4; ++u) dv[u] = nx[u];
         }
         const bool xl = x0 == 0, xr = x0 + 4 == W;  // border columns keep their label
         *reinterpret_cast<int4 *>(out + n0) =
@@ -315,6 +332,7 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
     if (e != hipSuccess || iters == 0 || W < 3 || H < 3) return e;
     const bool lds = L <= kLdsMaxLabels;
     const size_t lds_bytes = (size_t)L * ((L + 31) & ~31) * sizeof(T);
+    const size_t lds_bytes_x4 = (size_t)L * quad_row_stride(L) * sizeof(T);
     if (mode == GINF_ICM_SYNC) {
         int32_t *tmp = nullptr;
         if ((e = hipMallocAsync((void **)&tmp, nl, st)) != hipSuccess) return e;
@@ -336,7 +354,7 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
         const dim3 grid((unsigned)((W - 2 + 255) / 256), (unsigned)(H - 2)), block(256);
         for (int t = 0; t < iters && e == hipSuccess; ++t) {
             if (x4)
-                hipLaunchKernelGGL(k_icm_sync_lds_x4<T>, dim3(lds_grid), dim3(bs), lds_bytes, st, data, table, W, H,
+                hipLaunchKernelGGL(k_icm_sync_lds_x4<T>, dim3(lds_grid), dim3(bs), lds_bytes_x4, st, data, table, W, H,
                                    L, bufs[t & 1], bufs[(t + 1) & 1]);
             else if (lds)
                 hipLaunchKernelGGL(k_icm_sync_lds<T>, dim3(lds_grid), dim3(1024), lds_bytes, st, data, table, W, H, L,
