@@ -14,10 +14,11 @@
 //     work on diagonal k while iteration t works on diagonal k+2: launch s
 //     updates diagonal s-2t for every iteration t at once, which reproduces
 //     the sequential result exactly.
-// Costs stay in the reference layout [l][y][x]: a wave reads 64 adjacent x
-// of one label plane per load.  The L x L table V(a,b) at a + L*b is staged
-// in LDS for synchronous ICM with L <= 128 and read through the cache
-// otherwise.
+// Synchronous ICM reads the costs in the reference layout [l][y][x] (a wave
+// reads adjacent x of one label plane per load); asynchronous ICM first makes
+// a node-major copy [y][x][l], since its launches visit one anti-diagonal of
+// nodes at a time.  The L x L table V(a,b) at a + L*b is staged in LDS for
+// synchronous ICM with L <= 128 and read through the cache otherwise.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -258,6 +259,11 @@ __global__ __launch_bounds__(768) void k_icm_sync_lds_x4(const T *__restrict__ d
 // and a step's few nodes now wait for L/16 table reads each instead of L.
 constexpr int kAsyncLanes = 16;
 
+// Costs come from the node-major copy made by k_costs_node_major (element
+// (l, n) at n * L + l), so the 16 lanes of a node read 64 contiguous bytes
+// per step; in the reference layout every node of a diagonal touched its own
+// cache line in every plane (1920x1080x128, 10 iterations: 33.5 ms with the
+// reference layout, 19.0 ms with the copy, whose one pass costs ~0.4 ms).
 template <typename T>
 __global__ __launch_bounds__(256) void k_icm_async_lanes(const T *__restrict__ data, const T *__restrict__ table,
                                                          int W, int H, int L, int32_t *__restrict__ lab, int s, int t0,
@@ -272,7 +278,6 @@ __global__ __launch_bounds__(256) void k_icm_async_lanes(const T *__restrict__ d
     const bool active = x <= xe;  // inactive lanes still join the shuffles
     const int y = k - x;
     const int64_t n = (int64_t)y * W + x;
-    const int64_t plane = (int64_t)W * H;
     T best = T(0);
     int arg = -1;
     if (active) {
@@ -281,7 +286,7 @@ __global__ __launch_bounds__(256) void k_icm_async_lanes(const T *__restrict__ d
         const T *vS = table + (int64_t)L * fS, *vE = table + (int64_t)L * fE;
         const T *vN = table + (int64_t)L * fN, *vW = table + (int64_t)L * fW;
         for (int l = q; l < L; l += kAsyncLanes) {
-            T c = data[(int64_t)l * plane + n];
+            T c = data[n * L + l];
             c = c + vS[l];
             c = c + vE[l];
             c = c + vN[l];
@@ -296,6 +301,30 @@ __global__ __launch_bounds__(256) void k_icm_async_lanes(const T *__restrict__ d
         if (oa >= 0 && (arg < 0 || ob < best || (!(best < ob) && oa < arg))) { best = ob; arg = oa; }
     }
     if (active && q == 0) lab[n] = arg;
+}
+
+// Node-major copy of the costs for asynchronous ICM: out[n * L + l] =
+// data[l * P + n] (P = W * H), through a 32-label x 64-node LDS tile so that
+// both the plane reads and the node-row writes are contiguous.
+template <typename T>
+__global__ __launch_bounds__(256) void k_costs_node_major(const T *__restrict__ data, int64_t P, int L,
+                                                          T *__restrict__ out) {
+    __shared__ T tile[32][65];
+    const int64_t n0 = (int64_t)blockIdx.x * 64;
+    const int l0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+    for (int r = ty; r < 32; r += 4) {
+        const int l = l0 + r;
+        const int64_t n = n0 + tx;
+        if (l < L && n < P) tile[r][tx] = data[(int64_t)l * P + n];
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 31, ny = threadIdx.x >> 5;  // 32 x 8
+    for (int c = ny; c < 64; c += 8) {
+        const int64_t n = n0 + c;
+        const int l = l0 + lx;
+        if (l < L && n < P) out[n * L + l] = tile[lx][c];
+    }
 }
 
 // ---- table in global memory (any L) ---------------------------------------
@@ -370,6 +399,13 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
     const int nd = (W - 2) + (H - 2) - 1;  // anti-diagonals of the interior
     const int longest = min(W - 2, H - 2);
     const int steps = nd + 2 * (iters - 1);
+    // Node-major copy of the costs (one read and one write of the volume).
+    const int64_t P = (int64_t)W * H;
+    T *nm = nullptr;
+    if ((e = hipMallocAsync((void **)&nm, (size_t)P * L * sizeof(T), st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_costs_node_major<T>, dim3((unsigned)((P + 63) / 64), (unsigned)((L + 31) / 32)), dim3(256),
+                       0, st, data, P, L, nm);
+    e = hipGetLastError();
     for (int s = 0; s < steps && e == hipSuccess; ++s) {
         // iterations with 0 <= s - 2t < nd
         const int t_lo = s < nd ? 0 : (s - nd) / 2 + 1;
@@ -379,10 +415,11 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
         // to repay staging it in every block; measured 2x slower with LDS)
         const int64_t lanes = (int64_t)longest * kAsyncLanes;
         const dim3 grid((unsigned)((lanes + 255) / 256), (unsigned)(t_hi - t_lo + 1)), block(256);
-        hipLaunchKernelGGL(k_icm_async_lanes<T>, grid, block, 0, st, data, table, W, H, L, result, s, t_lo, nd);
+        hipLaunchKernelGGL(k_icm_async_lanes<T>, grid, block, 0, st, nm, table, W, H, L, result, s, t_lo, nd);
         e = hipGetLastError();
     }
-    return e;
+    const hipError_t f = hipFreeAsync(nm, st);
+    return e != hipSuccess ? e : f;
 }
 
 }  // namespace
