@@ -258,12 +258,14 @@ __global__ __launch_bounds__(768) void k_icm_sync_lds_x4(const T *__restrict__ d
 // to the lower label.  That is the sequential scan's result for finite costs,
 // and a step's few nodes now wait for L/16 table reads each instead of L.
 constexpr int kAsyncLanes = 16;
+constexpr int kAsyncChunk = 8;  // labels per lane per round of loads (L <= 128: one round)
 
 // Costs come from the node-major copy made by k_costs_node_major (element
 // (l, n) at n * L + l), so the 16 lanes of a node read 64 contiguous bytes
 // per step; in the reference layout every node of a diagonal touched its own
 // cache line in every plane (1920x1080x128, 10 iterations: 33.5 ms with the
-// reference layout, 19.0 ms with the copy, whose one pass costs ~0.4 ms).
+// reference layout, 18.9 ms with the copy, whose one pass costs ~0.5 ms, and
+// 14.5 ms with each chunk's loads issued together).
 template <typename T>
 __global__ __launch_bounds__(256) void k_icm_async_lanes(const T *__restrict__ data, const T *__restrict__ table,
                                                          int W, int H, int L, int32_t *__restrict__ lab, int s, int t0,
@@ -285,13 +287,32 @@ __global__ __launch_bounds__(256) void k_icm_async_lanes(const T *__restrict__ d
         const int fN = clamp_label(lab[n - W], L), fW = clamp_label(lab[n - 1], L);
         const T *vS = table + (int64_t)L * fS, *vE = table + (int64_t)L * fE;
         const T *vN = table + (int64_t)L * fN, *vW = table + (int64_t)L * fW;
-        for (int l = q; l < L; l += kAsyncLanes) {
-            T c = data[n * L + l];
-            c = c + vS[l];
-            c = c + vE[l];
-            c = c + vN[l];
-            c = c + vW[l];
-            if (arg < 0 || c < best) { best = c; arg = l; }
+        // Labels q, q+16, ... in chunks of kAsyncChunk whose cost and table
+        // loads are all issued before the chunk's sums (indices past L - 1
+        // re-read label L - 1 and are not compared).
+        for (int l0 = q; l0 < L; l0 += kAsyncChunk * kAsyncLanes) {
+            T dv[kAsyncChunk], tS[kAsyncChunk], tE[kAsyncChunk], tN[kAsyncChunk], tW[kAsyncChunk];
+#pragma unroll
+            for (int u = 0; u < kAsyncChunk; ++u) {
+                const int l = min(l0 + u * kAsyncLanes, L - 1);
+                dv[u] = data[n * L + l];
+                tS[u] = vS[l];
+                tE[u] = vE[l];
+                tN[u] = vN[l];
+                tW[u] = vW[l];
+            }
+#pragma unroll
+            for (int u = 0; u < kAsyncChunk; ++u) {
+                const int l = l0 + u * kAsyncLanes;
+                if (l < L) {
+                    T c = dv[u];
+                    c = c + tS[u];
+                    c = c + tE[u];
+                    c = c + tN[u];
+                    c = c + tW[u];
+                    if (arg < 0 || c < best) { best = c; arg = l; }
+                }
+            }
         }
     }
 #pragma unroll
