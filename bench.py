@@ -225,10 +225,22 @@ def bench_icm(args) -> dict:
 
     runs = max(5, args.steps)
     evals = (W - 2) * (H - 2) * L * N
+    if args.copy_mib:  # profiler counter calibration: one copy of known size
+        src = torch.ones(args.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     sync_ms, data, init, _ = gpu_run(W, H, G.ICM_SYNC, runs)
     async_ms, _, _, _ = gpu_run(W, H, G.ICM_ASYNC, max(3, runs // 2))
     del data, init
     algo = (W - 2) * (H - 2) * (4 * L + 8) * N
+    # HBM bytes per synchronous iteration, measured with rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE (scaled by a 1 GiB copy) on this workload; see the profile summary.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "icm_sync_1920x1080x128_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
     # CPU sample and its parity check
     Ws, Hs = 960, 540
     _, sd, sinit, sout = gpu_run(Ws, Hs, G.ICM_SYNC, 1)
@@ -249,7 +261,7 @@ def bench_icm(args) -> dict:
                    "sync_ms": round(sync_ms, 4), "async_ms": round(async_ms, 4),
                    "async_launches": W + H + 2 * N, "labels_equal_oracle_at_960x540": same},
         "roofline": {"bound": "hbm", "achieved": round(algo / sync_ms / 1e6, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(algo / sync_ms / 1e6 / 8000.0, 4), "traffic": None,
+                     "frac": round(algo / sync_ms / 1e6 / 8000.0, 4), "traffic": traffic,
                      "kernel": "synchronous ICM, 10 launches (4 B per node-label + 8 B per node and iteration)"},
         "cpu_baseline": {"value": round(cpu_evals / cpu_s / 1e9, 5), "unit": "Gnode-label evaluations/s", "cores": 1,
                          "kind": "port", "sample": "the same recipe at 960x540, 10 synchronous iterations, one run",
