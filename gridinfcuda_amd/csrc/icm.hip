@@ -373,6 +373,41 @@ __global__ void k_max_prior(const T *__restrict__ data, int W, int H, int L, int
     }
 }
 
+// Four nodes per thread (W * H % 4 == 0, 16-byte aligned buffers): one
+// 16-byte load per label brings four nodes' costs, and a group of eight
+// labels' loads is issued before their comparisons (labels past L - 1
+// re-read label L - 1 and are not compared).  Per node the scan is that of
+// k_max_prior: lowest-index label of minimal cost.
+template <typename T>
+__global__ __launch_bounds__(256) void k_max_prior_x4(const T *__restrict__ data, int64_t n, int L,
+                                                      int32_t *__restrict__ out) {
+    using V4 = typename Vec4<T>::type;
+    constexpr int U = 8;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < (n >> 2); q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = 4 * q;
+        const V4 b0 = *reinterpret_cast<const V4 *>(data + i);
+        T best[4] = {b0.x, b0.y, b0.z, b0.w};
+        int arg[4] = {0, 0, 0, 0};
+        for (int l0 = 1; l0 < L; l0 += U) {
+            V4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const V4 *>(data + (int64_t)min(l0 + u, L - 1) * n + i);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int l = l0 + u;
+                if (l < L) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const T c = lane_of<T>(v[u], k);
+                        if (c < best[k]) { best[k] = c; arg[k] = l; }
+                    }
+                }
+            }
+        }
+        *reinterpret_cast<int4 *>(out + i) = int4{arg[0], arg[1], arg[2], arg[3]};
+    }
+}
+
 template <typename T>
 hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32_t *initial, int iters, int mode,
                  int32_t *result, hipStream_t st) {
@@ -474,6 +509,17 @@ int ginf_max_prior(int width, int height, int num_labels, int dtype, const void 
                    void *stream) {
     if (!data_dev || !result_dev || width < 1 || height < 1 || num_labels < 1) return GINF_ERR_INVALID;
     const int64_t n = (int64_t)width * height;
+    if (n % 4 == 0 && ((uintptr_t)data_dev & 15) == 0 && ((uintptr_t)result_dev & 15) == 0 &&
+        (dtype == GINF_F32 || dtype == GINF_I32)) {
+        const unsigned qb = (unsigned)std::min<int64_t>((n / 4 + 255) / 256, 65536);
+        if (dtype == GINF_F32)
+            hipLaunchKernelGGL(k_max_prior_x4<float>, dim3(qb), dim3(256), 0, (hipStream_t)stream,
+                               (const float *)data_dev, n, num_labels, result_dev);
+        else
+            hipLaunchKernelGGL(k_max_prior_x4<int>, dim3(qb), dim3(256), 0, (hipStream_t)stream, (const int *)data_dev,
+                               n, num_labels, result_dev);
+        return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+    }
     const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
     if (dtype == GINF_F32)
         hipLaunchKernelGGL(k_max_prior<float>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const float *)data_dev,

@@ -75,9 +75,11 @@ def test_icm_sync_unaligned_buffers(oracle_mod, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.float32])
-def test_max_prior_matches_oracle(oracle_mod, dtype):
+@pytest.mark.parametrize("shape", [(12, 33, 47), (12, 32, 47), (1, 8, 6), (19, 4, 1)])
+def test_max_prior_matches_oracle(oracle_mod, dtype, shape):
+    """W*H % 4 == 0 runs four nodes per thread (second to fourth shapes)."""
     rng = np.random.default_rng(9)
-    data = rng.integers(0, 5, size=(12, 33, 47)).astype(dtype)
+    data = rng.integers(0, 5, size=shape).astype(dtype)
     np.testing.assert_array_equal(G.max_prior_host(data), oracle_mod.max_prior(data))
 
 
