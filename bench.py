@@ -77,6 +77,19 @@ def cpu_baseline(threads_all: int) -> dict:
     }
 
 
+def _interior_updates(W: int, H: int, L: int, iters: int) -> float:
+    """Node-label updates of plain BP (one level): interior nodes of the
+    half-sweep's colour, times L, summed over the half-sweeps."""
+    if W < 3 or H < 3:
+        return 0.0
+    ys, xs = np.mgrid[1:H - 1, 1:W - 1]
+    par = (xs + ys) & 1
+    odd = float(par.sum())
+    even = float(par.size) - odd
+    # half-sweep i updates (x + y + i) odd: i even -> odd parity, i odd -> even
+    return L * (odd * ((iters + 1) // 2) + even * (iters // 2))
+
+
 def bench_batched(args) -> dict:
     """Batched plain BP: 64 instances (widths log-uniform 160..1280, heights
     120..720, labels from {16,32,64,128}, Potts w=15 / TL lambda=8 tau=4
@@ -98,6 +111,11 @@ def bench_batched(args) -> dict:
         entry = autotune.tune_batch(bt, dp, lp, reps=3, path=autotune.CACHE if args.retune else None,
                                     log=lambda m: print(m, file=sys.stderr))
         tuned = "searched now"
+    if args.copy_mib:  # profiler counter calibration: one copy of known size
+        src = torch.ones(args.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     for _ in range(max(1, args.warmup)):
         bt.run(dp, lp, sp)
     torch.cuda.synchronize()
@@ -108,6 +126,35 @@ def bench_batched(args) -> dict:
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1000.0 / args.steps
     U = bt.updates_per_run()
+    # HBM bytes per graph replay, measured with rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE (scaled by a 1 GiB copy) on this workload; see the profile summary.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "batched_64mrf_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_replay"]
+    cpu = None
+    if not args.no_cpu:
+        # The oracle, one thread, on the batch's first instances up to ~1.5e8
+        # node-label updates (a bounded sample; the mean normalisation the
+        # batch runs), with their labels checked against the replay's.
+        from oracle import oracle as O
+        done, secs, same, names = 0.0, 0.0, True, []
+        for k, x in enumerate(insts):
+            u = _interior_updates(x["W"], x["H"], x["L"], 20)
+            if done and done + u > 1.5e8:
+                break
+            d = B.instance_costs(x)
+            t0 = time.perf_counter()
+            ref = O.hbp(d, x["table"], x["model"], 1, 20, norm=0)
+            secs += time.perf_counter() - t0
+            done += u
+            same = same and bool(np.array_equal(ref, labels[k].cpu().numpy()))
+            names.append(f"{x['W']}x{x['H']}x{x['L']}")
+        cpu = {"value": done / secs / 1e9, "unit": "Gnode-label msg updates/s", "cores": 1, "kind": "port",
+               "sample": f"the oracle on the batch's first {len(names)} instances ({', '.join(names)}), "
+                         f"20 half-sweeps each, {done:.4g} node-label updates, one run",
+               "labels_equal_gpu": same, "seconds": round(secs, 3), "cpu_model": cpu_model()}
     value = U / (ms / 1000.0) / 1e9
     # the generator's costs are integers in [0, 40]: every instance keeps its
     # data costs in 16-bit storage (34 B per update)
@@ -123,10 +170,10 @@ def bench_batched(args) -> dict:
                    "label_hist": {str(c): sum(1 for x in insts if x["L"] == c) for c in B.LABEL_CLASSES},
                    "tiles": {"slots_per_item": list(bt.tiles[0]), "blocks_per_cu": bt.tiles[1], "source": tuned}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "whole graph (k_sweep_batch16 + relayouts), 34 B per node-label update "
-                               "(16-bit data costs)"},
-        "cpu_baseline": None,
+                               "(16-bit data costs); traffic per replay"},
+        "cpu_baseline": cpu,
     }
 
 
