@@ -158,6 +158,23 @@ __global__ void k_ab_label(const int32_t *__restrict__ f, int32_t *__restrict__ 
         out[n] = g.in[n] ? (g.h[n] < g.inf ? alpha : beta) : f[n];
 }
 
+// Keep-or-reject of a pair's move on the device (graphcut.cc:263-272): the
+// move is kept when the new labelling cost is lower; *keep tells k_ab_commit
+// whether to copy the candidate labels over the current ones.
+__global__ void k_ab_decide(const double *__restrict__ en, double *__restrict__ best, int *__restrict__ moves,
+                            int *__restrict__ keep) {
+    const bool k = *en < *best;
+    if (k) { *best = *en; *moves += 1; }
+    *keep = k ? 1 : 0;
+}
+
+__global__ void k_ab_commit(const int *__restrict__ keep, const int32_t *__restrict__ cand, int32_t *__restrict__ cur,
+                            int64_t N) {
+    if (*keep == 0) return;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (int64_t)gridDim.x * blockDim.x)
+        cur[n] = cand[n];
+}
+
 struct Ctx {
     hipStream_t st;
     unsigned blocks;
@@ -250,9 +267,11 @@ int ginf_alpha_beta(int width, int height, int num_labels, int dtype, const void
     const int eblocks = 256;
     hipError_t e = hipMalloc((void **)&pool, sizeof(int) * (size_t)(9 * N + 2));
     if (e == hipSuccess) e = hipMalloc((void **)&cand, sizeof(int32_t) * (size_t)N);
-    if (e == hipSuccess) e = hipMalloc((void **)&ebuf, sizeof(double) * (eblocks + 1));
+    // ebuf: eblocks partial sums, the candidate's energy, the best energy, then
+    // two ints (kept moves, keep flag of the last pair)
+    if (e == hipSuccess) e = hipMalloc((void **)&ebuf, sizeof(double) * (eblocks + 3));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c.host, 2 * sizeof(int), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&ehost, sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&ehost, sizeof(double) + 2 * sizeof(int), hipHostMallocDefault);
     Flow g{};
     g.W = width;
     g.H = height;
@@ -266,46 +285,51 @@ int ginf_alpha_beta(int width, int height, int num_labels, int dtype, const void
         g.r = pool + 5 * N;
         g.counter = pool + 9 * N;
     }
-    int moves = 0;
-    auto energy = [&](const int32_t *lab, double *out) -> hipError_t {
-        hipError_t ee = ginf::launch_energy(GINF_I32, data_dev, table_dev, lab, width, height, num_labels, ebuf,
-                                            eblocks, ebuf + eblocks, st);
-        if (ee == hipSuccess) ee = hipMemcpyAsync(ehost, ebuf + eblocks, sizeof(double), hipMemcpyDeviceToHost, st);
-        if (ee == hipSuccess) ee = hipStreamSynchronize(st);
-        *out = *ehost;
-        return ee;
+    double *en_dev = ebuf ? ebuf + eblocks : nullptr, *best_dev = ebuf ? ebuf + eblocks + 1 : nullptr;
+    int *moves_dev = ebuf ? reinterpret_cast<int *>(ebuf + eblocks + 2) : nullptr;
+    int *keep_dev = moves_dev ? moves_dev + 1 : nullptr;
+    auto energy = [&](const int32_t *lab) -> hipError_t {
+        return ginf::launch_energy(GINF_I32, data_dev, table_dev, lab, width, height, num_labels, ebuf, eblocks, en_dev,
+                                   st);
     };
     if (e == hipSuccess && result_dev != initial_dev)
         e = hipMemcpyAsync(result_dev, initial_dev, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToDevice, st);
     double best = 0.0;
-    if (e == hipSuccess) e = energy(result_dev, &best);
+    if (e == hipSuccess) e = energy(result_dev);
+    if (e == hipSuccess) e = hipMemcpyAsync(best_dev, en_dev, sizeof(double), hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ehost, en_dev, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemsetAsync(moves_dev, 0, 2 * sizeof(int), st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) best = *ehost;
     const bool bad_labels = e == hipSuccess && best != best;  // the energy is NaN for a label outside [0, L)
     // Push-relabel needs O(N^2) rounds at worst; this bound only stops a
     // runaway loop.
     const int64_t max_rounds = std::max<int64_t>(4096, 8 * N);
-    int32_t *cur = result_dev, *nxt = cand;
+    // The current labelling stays in result_dev: a kept move copies the
+    // candidate over it on the device, so the host waits for no energy.
+    int moves = 0;
     bool success = true;
+    int *moves_host = reinterpret_cast<int *>(ehost + 1);
     for (int t = 0; t < iters && success && e == hipSuccess && !bad_labels; ++t) {
-        success = false;
+        const int before = moves;
         for (int a = 0; a < num_labels && e == hipSuccess; ++a)
             for (int b = a + 1; b < num_labels && e == hipSuccess; ++b) {
                 hipLaunchKernelGGL(k_ab_build, dim3(c.blocks), dim3(256), 0, st, (const int *)data_dev,
-                                   (const int *)table_dev, num_labels, cur, a, b, g);
+                                   (const int *)table_dev, num_labels, result_dev, a, b, g);
                 if ((e = hipGetLastError()) != hipSuccess) break;
                 if ((e = max_preflow(c, g, max_rounds)) != hipSuccess) break;
-                hipLaunchKernelGGL(k_ab_label, dim3(c.blocks), dim3(256), 0, st, cur, nxt, g, a, b);
-                double en = 0.0;
-                if ((e = energy(nxt, &en)) != hipSuccess) break;
-                if (en < best) {
-                    best = en;
-                    std::swap(cur, nxt);
-                    success = true;
-                    ++moves;
-                }
+                hipLaunchKernelGGL(k_ab_label, dim3(c.blocks), dim3(256), 0, st, result_dev, cand, g, a, b);
+                if ((e = energy(cand)) != hipSuccess) break;
+                hipLaunchKernelGGL(k_ab_decide, dim3(1), dim3(1), 0, st, en_dev, best_dev, moves_dev, keep_dev);
+                hipLaunchKernelGGL(k_ab_commit, dim3(c.blocks), dim3(256), 0, st, keep_dev, cand, result_dev, N);
+                e = hipGetLastError();
             }
+        // one read of the kept-move count per round over the pairs
+        if (e == hipSuccess) e = hipMemcpyAsync(moves_host, moves_dev, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e == hipSuccess) moves = *moves_host;
+        success = moves > before;
     }
-    if (e == hipSuccess && cur != result_dev)
-        e = hipMemcpyAsync(result_dev, cur, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (pool) (void)hipFree(pool);
     if (cand) (void)hipFree(cand);
