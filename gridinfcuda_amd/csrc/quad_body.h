@@ -7,7 +7,9 @@
 
 // Cache policy of the streaming buffer accesses (0 = default; 2 = non-temporal).
 // Outboxes are not read again within the half-sweep, so they are stored
-// non-temporally (measured faster on 1920x1080x128); loads keep the default.
+// non-temporally (measured faster on 1920x1080x128); loads keep the default
+// (1920x1080x128 HBP per decode: default 11.41-11.44 ms, non-temporal loads
+// 11.73 ms, policy 1 loads 11.41 ms).
 #ifndef GINF_QUAD_LOAD_POLICY
 #define GINF_QUAD_LOAD_POLICY 0
 #endif
