@@ -74,24 +74,39 @@ __global__ void k_clear(int *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
-// One in-place relaxation step of the BFS distances to the sink.
+// In-place relaxation steps of the BFS distances to the sink: GINF_AB_BFS_INNER
+// sweeps over the nodes per launch, separated by block barriers.  Heights only
+// decrease and every value read is an upper bound of the distance, so any
+// interleaving converges to the same fixed point (the exact distances); a
+// launch that changes nothing has reached it.
+// 384x288x16, one cycle, alternating runs on one box: 1 sweep per launch
+// 49.9-51.9 ms, 2 sweeps 43.8 (batches of 4), 4 sweeps 40.8-46.1 (batches of
+// 2), 46.4 (batches of 4), 52.1 (batches of 1), 8 sweeps 51.2 (batches of 1).
+#ifndef GINF_AB_BFS_INNER
+#define GINF_AB_BFS_INNER 4
+#endif
 __global__ void k_ab_bfs(Flow g) {
     const int64_t N = (int64_t)g.W * g.H;
     int changed = 0;
-    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (int64_t)gridDim.x * blockDim.x) {
-        if (!g.in[n]) continue;
-        const int x = (int)(n % g.W), y = (int)(n / g.W);
-        const int cur = g.h[n];
-        int best = cur;
-        for (int d = 0; d < 4; ++d) {
-            const int nx = x + kDX[d], ny = y + kDY[d];
-            if (nx < 0 || ny < 0 || nx >= g.W || ny >= g.H || g.r[4 * n + d] <= 0) continue;
-            const int hv = __hip_atomic_load(&g.h[(int64_t)ny * g.W + nx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (hv < g.inf && hv + 1 < best) best = hv + 1;
-        }
-        if (best < cur) {
-            __hip_atomic_store(&g.h[n], best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            changed = 1;
+    for (int it = 0; it < GINF_AB_BFS_INNER; ++it) {
+        if (it) __syncthreads();
+        for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N;
+             n += (int64_t)gridDim.x * blockDim.x) {
+            if (!g.in[n]) continue;
+            const int x = (int)(n % g.W), y = (int)(n / g.W);
+            const int cur = __hip_atomic_load(&g.h[n], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int best = cur;
+            for (int d = 0; d < 4; ++d) {
+                const int nx = x + kDX[d], ny = y + kDY[d];
+                if (nx < 0 || ny < 0 || nx >= g.W || ny >= g.H || g.r[4 * n + d] <= 0) continue;
+                const int hv =
+                    __hip_atomic_load(&g.h[(int64_t)ny * g.W + nx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (hv < g.inf && hv + 1 < best) best = hv + 1;
+            }
+            if (best < cur) {
+                __hip_atomic_store(&g.h[n], best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                changed = 1;
+            }
         }
     }
     if (changed) atomicAdd(&g.counter[1], 1);
@@ -187,13 +202,14 @@ hipError_t read_counters(Ctx &c, Flow &g) {
     return e;
 }
 
-// Launches between host checks of the convergence counters: BFS steps of
+// Launches between host checks of the convergence counters: BFS launches of
 // the global relabel, and push/relabel rounds of the maximum preflow.  The
-// in-place BFS converges in few steps, so its batch is short (384x288x16,
-// one cycle, with a counter clear before every step: 64.0 ms with batches
-// 4/8, 82.8-87.2 with 8/8, 92.7 with 2/4).
+// in-place BFS converges in few launches, so its batch is short (384x288x16,
+// one cycle, one sweep per BFS launch and a counter clear before every
+// launch: 64.0 ms with batches 4/8, 82.8-87.2 with 8/8, 92.7 with 2/4; with
+// four sweeps per BFS launch, batches of 2 BFS launches measured best).
 #ifndef GINF_AB_BFS_BATCH
-#define GINF_AB_BFS_BATCH 4
+#define GINF_AB_BFS_BATCH 2
 #endif
 #ifndef GINF_AB_ROUNDS
 #define GINF_AB_ROUNDS 8
