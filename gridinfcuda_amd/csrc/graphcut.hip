@@ -340,7 +340,7 @@ int ginf_alpha_beta(int width, int height, int num_labels, int dtype, const void
                 hipLaunchKernelGGL(k_ab_commit, dim3(c.blocks), dim3(256), 0, st, keep_dev, cand, result_dev, N);
                 e = hipGetLastError();
             }
-        // one read of the kept-move count per round over the pairs
+        // one read of the kept-move count per cycle over the pairs
         if (e == hipSuccess) e = hipMemcpyAsync(moves_host, moves_dev, sizeof(int), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e == hipSuccess) moves = *moves_host;
