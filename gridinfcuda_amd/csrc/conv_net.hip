@@ -228,7 +228,7 @@ __global__ void k_flip_weights(const uint16_t *__restrict__ w, int Cout, int Cin
 // L but the output is f*f*2 times smaller than the input.
 template <typename T>
 __global__ void k_minpool(const T *__restrict__ d, int W, int H, int L, int f, float scale, int w4, int h4,
-                          uint16_t *__restrict__ out) {
+                          uint16_t *__restrict__ out, bool vec4) {
     const int64_t n = (int64_t)w4 * h4 * L;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int X = (int)(i % w4);
@@ -238,11 +238,25 @@ __global__ void k_minpool(const T *__restrict__ d, int W, int H, int L, int f, f
         const int x1 = min(X * f + f, W), y1 = min(Y * f + f, H);
         const T *plane = d + (int64_t)l * H * W;
         float mn = 3.4e38f;
-        for (int yy = Y * f; yy < y1; ++yy)
-            for (int xx = X * f; xx < x1; ++xx) {
-                const float v = (float)plane[(int64_t)yy * W + xx];
-                mn = v < mn ? v : mn;
+        if (f == 4 && x1 - X * f == 4 && vec4) {
+            // whole 4-wide row segments, 16-byte aligned: one load per row,
+            // the same left-to-right comparisons
+            typedef T t4 __attribute__((ext_vector_type(4)));
+            for (int yy = Y * f; yy < y1; ++yy) {
+                const t4 v4 = *reinterpret_cast<const t4 *>(plane + (int64_t)yy * W + X * f);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float v = (float)v4[k];
+                    mn = v < mn ? v : mn;
+                }
             }
+        } else {
+            for (int yy = Y * f; yy < y1; ++yy)
+                for (int xx = X * f; xx < x1; ++xx) {
+                    const float v = (float)plane[(int64_t)yy * W + xx];
+                    mn = v < mn ? v : mn;
+                }
+        }
         out[((int64_t)Y * w4 + X) * L + l] = f2bf(mn * scale);
     }
 }
@@ -276,6 +290,7 @@ struct InjectArgs {
     ginf::LevelGeom g;
     int C, G, interleave, L, shift, pw, ph;
     float inv_scale;
+    int pred16;  // prediction rows 16-byte aligned (L % 4 == 0, aligned base)
 };
 
 // One thread per (register plane, slot) of the outbox array of colour
@@ -300,12 +315,22 @@ __global__ void k_inject(const float *__restrict__ pred, InjectArgs a) {
         if (a.interleave && G == 4 && ii * 4 + 3 < L) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (interior) {
-                v.x = p[ii * 4 + 0] * a.inv_scale;
-                v.y = p[ii * 4 + 1] * a.inv_scale;
-                v.z = p[ii * 4 + 2] * a.inv_scale;
-                v.w = p[ii * 4 + 3] * a.inv_scale;
+                if (a.pred16) {  // L % 4 == 0 and an aligned prediction: one 16-byte read
+                    v = *reinterpret_cast<const float4 *>(p + ii * 4);
+                } else {
+                    v.x = p[ii * 4 + 0];
+                    v.y = p[ii * 4 + 1];
+                    v.z = p[ii * 4 + 2];
+                    v.w = p[ii * 4 + 3];
+                }
+                v.x *= a.inv_scale;
+                v.y *= a.inv_scale;
+                v.z *= a.inv_scale;
+                v.w *= a.inv_scale;
             }
-            *(float4 *)o = v;
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const f4v w = {v.x, v.y, v.z, v.w};
+            __builtin_nontemporal_store(w, reinterpret_cast<f4v *>(o));  // not read again in this launch
         } else {
             for (int q = 0; q < G; ++q) {
                 const int l = a.interleave ? ii * G + q : q * C + ii;
@@ -430,12 +455,14 @@ int ginf_minpool_costs(const void *costs, int dtype, int W, int H, int L, int f,
     if (!costs || !out || f < 1) return GINF_ERR_INVALID;
     const int w4 = (W + f - 1) / f, h4 = (H + f - 1) / f;
     const int64_t n = (int64_t)w4 * h4 * L;
+    // 16-byte row segments: every row and plane starts 16-byte aligned
+    const bool vec4 = f == 4 && W % 4 == 0 && ((uintptr_t)costs & 15) == 0;
     if (dtype == GINF_F32)
         hipLaunchKernelGGL(k_minpool<float>, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, (const float *)costs, W,
-                           H, L, f, scale, w4, h4, out);
+                           H, L, f, scale, w4, h4, out, vec4);
     else
         hipLaunchKernelGGL(k_minpool<int>, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, (const int *)costs, W, H,
-                           L, f, scale, w4, h4, out);
+                           L, f, scale, w4, h4, out, vec4);
     return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
 }
 
@@ -474,6 +501,7 @@ hipError_t launch_inject(const float *pred, void *const msgs[2][4], const LevelG
     a.pw = pw;
     a.ph = ph;
     a.inv_scale = inv_scale;
+    a.pred16 = (L % 4 == 0 && ((uintptr_t)pred & 15) == 0) ? 1 : 0;
     if (C > 65535) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_inject, dim3(nblk(g.nslots), 8, C), dim3(256), 0, st, pred, a);
     return hipGetLastError();
