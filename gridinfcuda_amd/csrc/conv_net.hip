@@ -293,16 +293,18 @@ struct InjectArgs {
     int pred16;  // prediction rows 16-byte aligned (L % 4 == 0, aligned base)
 };
 
-// One thread per (register plane, slot) of the outbox array of colour
-// blockIdx.y & 1 and direction blockIdx.y >> 1 (plane blockIdx.z), writing
-// that slot's G labels contiguously (a 16-byte store for the quad layout), so
-// the writes of a wave (64 slots of one slot tile) are one contiguous run.  Border nodes get 0; padding
-// slots and padding labels are left as they are.
+// One thread per slot of the outbox array of colour blockIdx.y & 1 and
+// direction blockIdx.y >> 1, looping over the register planes: each store
+// writes that slot's G labels contiguously (a 16-byte store for the quad
+// layout), so the writes of a wave (64 slots of one slot tile) are one
+// contiguous run per plane, and the successive planes read successive 16 bytes
+// of the same prediction row (one cache line serves eight planes).  Border
+// nodes get 0; padding slots and padding labels are left as they are.
 __global__ void k_inject(const float *__restrict__ pred, InjectArgs a) {
     const int c = blockIdx.y & 1, d = blockIdx.y >> 1;
-    const int ii = blockIdx.z;
     float *__restrict__ out = (float *)a.msgs[c][d];
     const int G = a.G, C = a.C, L = a.L;
+    typedef float f4v __attribute__((ext_vector_type(4)));
     for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < a.g.nslots; slot += gridDim.x * blockDim.x) {
         const int y = slot / a.g.whp;
         if (y >= a.g.h) continue;
@@ -311,30 +313,30 @@ __global__ void k_inject(const float *__restrict__ pred, InjectArgs a) {
         const bool interior = x >= 1 && y >= 1 && x <= a.g.w - 2 && y <= a.g.h - 2;
         const int X = min(x >> a.shift, a.pw - 1), Y = min(y >> a.shift, a.ph - 1);
         const float *p = pred + ((int64_t)Y * a.pw + X) * 4 * L + d * L;
-        float *o = out + ginf::elem_index(slot, ii, 0, C, G);
-        if (a.interleave && G == 4 && ii * 4 + 3 < L) {
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (interior) {
-                if (a.pred16) {  // L % 4 == 0 and an aligned prediction: one 16-byte read
-                    v = *reinterpret_cast<const float4 *>(p + ii * 4);
-                } else {
-                    v.x = p[ii * 4 + 0];
-                    v.y = p[ii * 4 + 1];
-                    v.z = p[ii * 4 + 2];
-                    v.w = p[ii * 4 + 3];
+        for (int ii = 0; ii < C; ++ii) {
+            float *o = out + ginf::elem_index(slot, ii, 0, C, G);
+            if (a.interleave && G == 4 && ii * 4 + 3 < L) {
+                f4v v = {0.f, 0.f, 0.f, 0.f};
+                if (interior) {
+                    if (a.pred16) {  // L % 4 == 0 and an aligned prediction: one 16-byte read
+                        v = *reinterpret_cast<const f4v *>(p + ii * 4);
+                    } else {
+                        v[0] = p[ii * 4 + 0];
+                        v[1] = p[ii * 4 + 1];
+                        v[2] = p[ii * 4 + 2];
+                        v[3] = p[ii * 4 + 3];
+                    }
+                    v[0] *= a.inv_scale;
+                    v[1] *= a.inv_scale;
+                    v[2] *= a.inv_scale;
+                    v[3] *= a.inv_scale;
                 }
-                v.x *= a.inv_scale;
-                v.y *= a.inv_scale;
-                v.z *= a.inv_scale;
-                v.w *= a.inv_scale;
-            }
-            typedef float f4v __attribute__((ext_vector_type(4)));
-            const f4v w = {v.x, v.y, v.z, v.w};
-            __builtin_nontemporal_store(w, reinterpret_cast<f4v *>(o));  // not read again in this launch
-        } else {
-            for (int q = 0; q < G; ++q) {
-                const int l = a.interleave ? ii * G + q : q * C + ii;
-                if (l < L) o[q] = interior ? p[l] * a.inv_scale : 0.f;
+                __builtin_nontemporal_store(v, reinterpret_cast<f4v *>(o));  // not read again in this launch
+            } else {
+                for (int q = 0; q < G; ++q) {
+                    const int l = a.interleave ? ii * G + q : q * C + ii;
+                    if (l < L) o[q] = interior ? p[l] * a.inv_scale : 0.f;
+                }
             }
         }
     }
@@ -502,8 +504,7 @@ hipError_t launch_inject(const float *pred, void *const msgs[2][4], const LevelG
     a.ph = ph;
     a.inv_scale = inv_scale;
     a.pred16 = (L % 4 == 0 && ((uintptr_t)pred & 15) == 0) ? 1 : 0;
-    if (C > 65535) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_inject, dim3(nblk(g.nslots), 8, C), dim3(256), 0, st, pred, a);
+    hipLaunchKernelGGL(k_inject, dim3(nblk(g.nslots), 8), dim3(256), 0, st, pred, a);
     return hipGetLastError();
 }
 
