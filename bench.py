@@ -32,48 +32,82 @@ BYTES_PER_UPDATE = 36     # D (4) + 4 incoming (16) + 4 outgoing (16) FP32, per 
                           # (34 with 16-bit data costs)
 
 
-def cpu_model() -> str:
+def cpu_model() -> dict:
+    """Host CPU description from /proc/cpuinfo: model name, family/model/
+    stepping, the logical CPUs this process may run on and their MHz."""
+    info = {"model_name": platform.processor() or "unknown"}
     try:
         with open("/proc/cpuinfo") as f:
+            first = {}
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    if first:
+                        break
+                    continue
+                k, _, v = line.partition(":")
+                first[k.strip()] = v.strip()
+        info["model_name"] = first.get("model name", info["model_name"])
+        for key in ("vendor_id", "cpu family", "model", "stepping", "cpu MHz", "cache size"):
+            if key in first:
+                info[key.replace(" ", "_")] = first[key]
     except OSError:
         pass
-    return platform.processor() or "unknown"
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["usable_cpus"] = os.cpu_count()
+    info["os_cpu_count"] = os.cpu_count()
+    return info
 
 
-def cpu_baseline(threads_all: int) -> dict:
-    """The oracle (plain-C restatement of the reference CPU BP) on a bounded
-    sample of the same workload: the config recipe at 960x540 (1/4 of the
-    pixels), 128 labels, 5 levels x 10 half-sweeps."""
-    from oracle import oracle as O
-    from gridinfcuda_amd import _lib as G
-    W, H, L = 960, 540, 128
-    data = G.gen_stereo(2, W, H, L, 64, 127, 40)["cost_f32"]
-    table = O.table_trunc_linear(L, 8, 32)
-    # node-label updates of the sample: same counting as the GPU metric
-    plan_u = 0.0
-    w, h = W, H
-    for t in range(5):
+def _hbp_updates(W: int, H: int, L: int, levels: int, iters: int) -> float:
+    """Node-label updates of decodeHbp: interior nodes of each half-sweep's
+    colour at every level (ceil halving), times L."""
+    u, w, h = 0.0, W, H
+    for t in range(levels):
         if t:
             w, h = (w + 1) // 2, (h + 1) // 2
-        for i in range(10):
-            color = (i + 1) & 1
-            if w >= 3 and h >= 3:
-                ys, xs = np.mgrid[1:h - 1, 1:w - 1]
-                plan_u += float(((xs + ys) & 1 == color).sum()) * L
-    res = {}
-    for th in (1, threads_all):
+        u += _interior_updates(w, h, L, iters)
+    return u
+
+
+def cpu_baseline(threads_all: int, gpu_labels=None, runs_all: int = 3, log=None) -> dict:
+    """The oracle (plain-C restatement of the reference CPU BP, src/bp.cc:86-180,
+    built -O2 -m64 as the reference Makefile builds, Makefile:57,75-76) on the
+    benchmark workload itself: 1920x1080x128, 5 levels x 10 half-sweeps, mean
+    normalisation.  One run on one thread (the reference is single-threaded)
+    and the median of `runs_all` runs on `threads_all` threads (OpenMP across
+    the nodes of one colour, bit-identical to one thread).  The labels of every
+    run are compared with the GPU's."""
+    from oracle import oracle as O
+    from gridinfcuda_amd import _lib as G
+    log = log or (lambda m: print(m, file=sys.stderr, flush=True))
+    W, H, L = 1920, 1080, 128
+    data = G.gen_stereo(2, W, H, L, 64, 127, 40)["cost_f32"]
+    table = O.table_trunc_linear(L, 8, 32)
+    U = _hbp_updates(W, H, L, 5, 10)
+    times, same = {1: [], threads_all: []}, True
+    plan = [1] + [threads_all] * runs_all
+    for k, th in enumerate(plan):
+        log(f"cpu baseline: run {k + 1}/{len(plan)} on {th} thread(s)")
         t0 = time.perf_counter()
-        O.hbp(data, table, 2, 5, 10, norm=0, threads=th)
+        lab = O.hbp(data, table, 2, 5, 10, norm=0, threads=th)
         dt = time.perf_counter() - t0
-        res[th] = plan_u / dt / 1e9
+        times[th].append(dt)
+        log(f"cpu baseline: {dt:.1f} s")
+        if gpu_labels is not None:
+            same = same and bool(np.array_equal(lab, gpu_labels))
+    t1 = float(np.median(times[1]))
+    ta = float(np.median(times[threads_all]))
     return {
-        "value": res[1], "unit": "Gnode-label msg updates/s", "cores": 1, "kind": "port",
-        "sample": f"960x540x128 HBP 5 levels x 10 half-sweeps (config recipe at 1/4 of the pixels), "
-                  f"{plan_u:.4g} node-label updates, oracle built -O2 -m64",
-        "value_all_cores": res[threads_all], "cores_all": threads_all, "cpu_model": cpu_model(),
+        "value": U / t1 / 1e9, "unit": "Gnode-label msg updates/s", "cores": 1, "kind": "port",
+        "sample": f"the whole workload: 1920x1080x128 HBP 5 levels x 10 half-sweeps, {U:.4g} node-label updates, "
+                  f"oracle built -O2 -m64; 1 run on 1 thread, median of {runs_all} runs on {threads_all} threads",
+        "seconds_1_thread": [round(t, 2) for t in times[1]],
+        "value_all_cores": U / ta / 1e9, "cores_all": threads_all,
+        "seconds_all_cores": [round(t, 2) for t in times[threads_all]],
+        "labels_equal_gpu": same if gpu_labels is not None else None,
+        "cpu": cpu_model(),
     }
 
 
@@ -472,8 +506,10 @@ def bench_warmstart(args) -> dict:
     net = WS.ConvNet.load(WS.default_model_path(wl))
     seeds = list(range(2_000_000, 2_000_000 + args.eval_instances))
     log = lambda m: print(m, file=sys.stderr)
-    rows = {"net": WS.summarize(WS.evaluate(wl, net, seeds, reps=args.steps, log=log)),
-            "zero_seed": WS.summarize(WS.evaluate(wl, None, seeds, reps=args.steps, log=log))}
+    rec_net = WS.evaluate(wl, net, seeds, reps=args.steps, log=log)
+    rec_zero = WS.evaluate(wl, None, seeds, reps=args.steps, log=log)
+    rows = {"net": WS.summarize(rec_net), "zero_seed": WS.summarize(rec_zero)}
+    plain_ms = float(np.median([r["plain_ms"] for r in rec_net + rec_zero]))
 
     def best(rs):
         ok = [r for r in rs if r["within_0.1pct"] >= 1.0]
@@ -486,7 +522,7 @@ def bench_warmstart(args) -> dict:
         "data": f"synthetic (seeded stereo generator, seeds {seeds[0]}..{seeds[-1]})",
         "config": {"workload": "learned warm start 640x480x64 TL lambda=10 tau=3 vs plain HBP 5 levels x 10",
                    "instances": len(seeds), "target": "energy within 0.1% of plain HBP on every instance"},
-        "best_with_net": b_net, "best_zero_seed": b_zero, "schedules": rows,
+        "best_with_net": b_net, "best_zero_seed": b_zero, "plain_ms": round(plain_ms, 4), "schedules": rows,
         "roofline": None, "cpu_baseline": None,
     }
 
@@ -529,6 +565,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-api", action="store_true", help="skip the host-entry (PCIe-inclusive) timing")
     ap.add_argument("--no-probe", action="store_true", help="skip the HBM streaming probe")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload: skip the warm-start and batched figures added to the line")
     ap.add_argument("--data32", action="store_true",
                     help="keep 32-bit data costs (no 16-bit storage even when the costs fit)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
@@ -708,9 +746,32 @@ def main() -> int:
                                     "labels_equal_device_run": bool(np.array_equal(hl, labels.cpu().numpy()))}
         G.lib().ginf_hbp_host_release()
         del host
+    gpu_labels = labels.cpu().numpy()
+    del plan, data, labels, tdev
+    torch.cuda.empty_cache()
+    if not args.no_extras and default_shape:
+        # The benchmark's other two figures on the same line: the learned warm
+        # start (ms to plain-BP energy, warm start vs plain, 640x480x64 TL) and
+        # the batched workload (64 mixed MRFs).  Each is the whole record its
+        # own --workload run prints.
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        log("warm start: 4 instances of 640x480x64")
+        ws = bench_warmstart(argparse.Namespace(**{**vars(args), "eval_instances": 4, "steps": 5}))
+        out["warm_start"] = {
+            "metric": ws["metric"], "unit": ws["unit"], "instances": ws["config"]["instances"],
+            "best_with_net": ws["best_with_net"], "best_zero_seed": ws["best_zero_seed"],
+            "time_ratio_with_net": ws["best_with_net"]["time_ratio"] if ws["best_with_net"] else None,
+            "time_ratio_zero_seed": ws["best_zero_seed"]["time_ratio"] if ws["best_zero_seed"] else None,
+            "plain_ms": ws["plain_ms"], "schedules": ws["schedules"]}
+        torch.cuda.empty_cache()
+        log("batched: 64 mixed MRFs")
+        bb = bench_batched(argparse.Namespace(**{**vars(args), "steps": 5, "warmup": 2}))
+        out["batched"] = {k: bb[k] for k in ("metric", "value", "unit", "ms_per_step", "config", "roofline",
+                                             "cpu_baseline")}
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(threads)
+        out["cpu_baseline"] = cpu_baseline(threads, gpu_labels if default_shape else None)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

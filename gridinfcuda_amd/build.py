@@ -50,14 +50,28 @@ def _compile(src: str, build_dir: str = BUILD, extra=()) -> str:
     return obj
 
 
-def build(jobs: int | None = None, out: str = LIB, build_dir: str = BUILD, extra=()) -> str:
+def _headers_mtime() -> float:
+    names = [os.path.join(CSRC, n) for n in os.listdir(CSRC) if n.endswith((".h", ".inc"))]
+    names += [os.path.join(INCLUDE, n) for n in os.listdir(INCLUDE) if n.endswith(".h")]
+    return max((os.path.getmtime(n) for n in names), default=0.0)
+
+
+def build(jobs: int | None = None, out: str = LIB, build_dir: str = BUILD, extra=(), incremental: bool = False) -> str:
     """Compile every source and link `out`; `extra` adds compiler flags
-    (e.g. -D options for an experimental variant built to another path)."""
+    (e.g. -D options for an experimental variant built to another path).
+    incremental: keep objects newer than their source and every header."""
     os.makedirs(build_dir, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2)))
+    hdr = _headers_mtime()
+
+    def one(src):
+        obj = os.path.join(build_dir, os.path.basename(src) + ".o")
+        if incremental and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(src), hdr):
+            return obj
+        return _compile(src, build_dir, tuple(extra))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, build_dir, tuple(extra)), srcs))
+        objs = list(ex.map(one, srcs))
     tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -70,19 +84,25 @@ def build(jobs: int | None = None, out: str = LIB, build_dir: str = BUILD, extra
     return out
 
 
-def build_dropin_test() -> str:
-    """Link tests/cpp/dropin_stereo.cc (a C++ caller of the drop-in API)
-    against the library with the host compiler."""
-    src = os.path.join(ROOT, "tests", "cpp", "dropin_stereo.cc")
+def build_dropin_test() -> list[str]:
+    """Link the C++ callers of the drop-in API under tests/cpp/ against the
+    library with the host compiler, one executable each in
+    gridinfcuda_amd/bin/."""
+    cdir = os.path.join(ROOT, "tests", "cpp")
     os.makedirs(os.path.join(PKG, "bin"), exist_ok=True)
-    exe = os.path.join(PKG, "bin", "dropin_stereo")
-    cmd = ["g++", "-std=c++17", "-O2", f"-I{INCLUDE}", src, f"-L{PKG}", "-lginf_bp",
-           "-Wl,-rpath,$ORIGIN/..", "-o", exe]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("drop-in test link failed")
-    return exe
+    exes = []
+    for name in sorted(os.listdir(cdir)):
+        if not name.endswith(".cc"):
+            continue
+        exe = os.path.join(PKG, "bin", name[:-3])
+        cmd = ["g++", "-std=c++17", "-O2", f"-I{INCLUDE}", os.path.join(cdir, name), f"-L{PKG}", "-lginf_bp",
+               "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"drop-in test link failed: {name}")
+        exes.append(exe)
+    return exes
 
 
 if __name__ == "__main__":
@@ -90,6 +110,7 @@ if __name__ == "__main__":
     ap.add_argument("-j", type=int, default=None)
     ap.add_argument("-o", default=LIB, help="output library path")
     ap.add_argument("-D", action="append", default=[], help="extra preprocessor definition")
+    ap.add_argument("--incremental", action="store_true", help="recompile only sources newer than their objects")
     args = ap.parse_args()
     bdir = BUILD if args.o == LIB else os.path.join(os.path.dirname(os.path.abspath(args.o)), "_build_variant")
-    print(build(args.j, args.o, bdir, [f"-D{d}" for d in args.D]))
+    print(build(args.j, args.o, bdir, [f"-D{d}" for d in args.D], incremental=args.incremental))

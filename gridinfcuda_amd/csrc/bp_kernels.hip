@@ -246,7 +246,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_energy_partial(const T *data, const T *table, const int32_t *f, int W,
                                                         int H, int L, double *partials) {
     __shared__ double red[256];
-    const int64_t nin = (int64_t)(W - 2) * (H - 2);
+    // Grids without interior nodes (W < 3 or H < 3) have energy 0
+    // (src/grid.cc:102-125 sums over 1 <= x <= W-2, 1 <= y <= H-2); the clamp
+    // keeps (W-2)*(H-2) from turning positive for W = H = 1.
+    const int64_t nin = (W < 3 || H < 3) ? 0 : (int64_t)(W - 2) * (H - 2);
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nin; i += (int64_t)gridDim.x * blockDim.x) {
         const int y = 1 + (int)(i / (W - 2));

@@ -198,6 +198,14 @@ void gpuDecodeHbp(Grid<T> *grid, int numLevels, int numItersPerLevel, Matrix<int
     run_hbp("ginf::gpuDecodeHbp", grid, numLevels, numItersPerLevel, result);
 }
 
+void releaseBpDeviceState() {
+    const int rc = ginf_hbp_host_release();
+    if (rc != GINF_OK) {
+        fprintf(stderr, "ginf::releaseBpDeviceState failed (code %d): %s\n", rc, ginf_last_error());
+        exit(1);
+    }
+}
+
 template void decodeBp<float>(Grid<float> *, int, Matrix<int> *);
 template void decodeBp<int>(Grid<int> *, int, Matrix<int> *);
 template void gpuDecodeBp<float>(Grid<float> *, int, Matrix<int> *);

@@ -28,6 +28,14 @@ void decodeHbp(Grid<T> *grid, int numLevels, int numItersPerLevel, Matrix<int> *
 
 template <typename T>
 void gpuDecodeHbp(Grid<T> *grid, int numLevels, int numItersPerLevel, Matrix<int> *result);
+
+// No reference counterpart.  The four entry points above keep the device
+// state of the last problem they ran (its plan, costs and labels: about 8 GB
+// for 1920x1080 with 128 labels) so that a repeated call on the same problem
+// neither reallocates nor re-zeroes it; the reference frees its state at the
+// end of every call (src/bp.cc:176-179).  This frees it now; the next call
+// allocates afresh.
+void releaseBpDeviceState();
 }  // namespace ginf
 
 #endif

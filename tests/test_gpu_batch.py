@@ -12,10 +12,18 @@ pytestmark = pytest.mark.gpu
 
 
 def small_instances(n, seed):
+    """Instances 0-7 cycle through the label classes 16/32/64/128; 8-11 are
+    L = 65 and 96 (the capacity-96 kernel, full and partial) with Potts and
+    truncated linear each; the rest draw L at random."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
-        L = B.LABEL_CLASSES[i % 4] if i < 8 else int(rng.choice([5, 16, 17, 32, 50, 64, 100, 128]))
+        if i < 8:
+            L = B.LABEL_CLASSES[i % 4]
+        elif i < 12:
+            L = (65, 96, 96, 65)[i - 8]
+        else:
+            L = int(rng.choice([5, 16, 17, 32, 50, 64, 65, 96, 100, 128]))
         W, H = int(rng.integers(3, 70)), int(rng.integers(3, 50))
         potts = i % 2 == 0
         out.append(dict(index=i, seed=900 + i, W=W, H=H, L=L, model=G.SM_POTTS if potts else G.SM_TRUNC_LINEAR,
@@ -136,7 +144,7 @@ def test_batch_data_storage_forms(oracle_mod, compact):
     the oracle."""
     torch = torch_cuda()
     say(f"batch data storage compact={compact}")
-    insts = small_instances(8, 123)
+    insts = small_instances(12, 123)
     datas = [B.instance_costs(x) for x in insts]
     datas[2] = datas[2] + np.float32(0.25)
     datas[5] = datas[5].copy()

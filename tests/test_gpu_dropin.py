@@ -59,3 +59,59 @@ def test_cpp_dropin_refinement(oracle_mod, refine):
                                                                                                  sync=False)
     np.testing.assert_array_equal(labels, want)
     assert e_ref == oracle_mod.energy(data, table, want) <= e_bp
+
+
+MODELS_EXE = os.path.join(ROOT, "gridinfcuda_amd", "bin", "dropin_models")
+SM = dict(explicit=0, potts=1, tl=2, tq=3)
+
+
+@pytest.mark.parametrize("T,model,W,H,L,levels,iters,entry", [
+    ("int", "potts", 41, 29, 12, 3, 6, "hbp"),
+    ("float", "tl", 37, 23, 20, 1, 9, "bp"),
+    ("float", "potts", 33, 35, 8, 2, 5, "gpuhbp"),
+    ("int", "tq", 30, 22, 16, 2, 5, "hbp"),
+    ("float", "tq", 26, 19, 24, 1, 7, "gpubp"),
+    ("int", "explicit", 28, 20, 10, 1, 6, "bp"),
+    ("float", "explicit", 31, 17, 33, 3, 4, "hbp"),
+])
+def test_cpp_dropin_models(oracle_mod, tmp_path, T, model, W, H, L, levels, iters, entry):
+    """Grid<int> and Grid<float> through the C++ templates with every
+    smoothness setter (setSmoothnessCost selects the explicit table and the
+    O(L^2) kernel; useSmoothnessTruncQuad the same kernel with the quadratic
+    table), decoded twice with releaseBpDeviceState() in between: labels and
+    getLabelingCost equal the oracle's (float costs with a fractional part
+    included)."""
+    say(f"C++ drop-in models: Grid<{T}> {model} {W}x{H}x{L} levels={levels} iters={iters} {entry}")
+    assert os.path.exists(MODELS_EXE), "build the library first (python -m gridinfcuda_amd.build)"
+    dtype = np.int32 if T == "int" else np.float32
+    rng = np.random.default_rng(W * 1000 + H * 10 + L)
+    data = rng.integers(0, 40, size=(L, H, W)).astype(dtype)
+    if T == "float":
+        data = (data + rng.integers(0, 4, size=data.shape) * np.float32(0.25)).astype(np.float32)
+    a = np.arange(L)
+    if model == "potts":
+        table = (7 * (a[:, None] != a[None, :])).astype(dtype)
+    elif model == "tl":
+        table = np.minimum(3 * np.abs(a[:, None] - a[None, :]), 12).astype(dtype)
+    elif model == "tq":
+        table = np.minimum(2 * (a[:, None] - a[None, :]) ** 2, 30).astype(dtype)
+    else:
+        table = rng.integers(0, 20, size=(L, L)).astype(dtype)  # asymmetric
+    cpath, tpath = tmp_path / "costs.bin", tmp_path / "table.bin"
+    data.tofile(cpath)
+    # the oracle's table is V(a, b) at a + L*b, i.e. table[b, a] in numpy
+    np.ascontiguousarray(table).tofile(tpath)
+    out = subprocess.run([MODELS_EXE, T, model, str(W), str(H), str(L), str(levels), str(iters), str(cpath),
+                          str(tpath), entry], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.strip().split("\n")
+    labels = np.array([[int(v) for v in ln.split()] for ln in lines[:-1]], np.int32)
+    lv = 1 if entry in ("bp", "gpubp") else levels
+    ref = oracle_mod.hbp(data, table, SM[model], lv, iters, norm=0)
+    np.testing.assert_array_equal(labels, ref)
+    e = lines[-1].split()[1]
+    want = oracle_mod.energy(data, table, ref)
+    if T == "int":
+        assert int(e) == want
+    else:
+        assert np.float32(float(e)) == np.float32(want)

@@ -23,8 +23,12 @@ def sym_table(rng, L, kind):
 
 @pytest.mark.parametrize("kind", ["potts", "tl", "random"])
 @pytest.mark.parametrize("L,H,W,iters", [(3, 3, 3, 2), (4, 9, 13, 3), (6, 24, 32, 4), (5, 2, 17, 2), (8, 31, 7, 2),
-                                         (1, 5, 5, 3), (4, 12, 12, 0)])
+                                         (1, 5, 5, 3), (4, 12, 12, 0), (3, 1, 1, 2), (4, 1, 9, 2), (3, 2, 2, 2),
+                                         (4, 7, 2, 2)])
 def test_alpha_beta_matches_oracle(oracle_mod, kind, L, H, W, iters):
+    """Includes grids without interior nodes (1x1, 1xN, 2x2, Nx2): their
+    labelling cost is 0 for every labelling, so no move is ever kept and the
+    initial labels come back unchanged."""
     torch = torch_cuda()
     say(f"alpha-beta {kind} L={L} {W}x{H} iters={iters}")
     rng = np.random.default_rng(L * 1000 + H * 10 + W)
@@ -39,6 +43,9 @@ def test_alpha_beta_matches_oracle(oracle_mod, kind, L, H, W, iters):
     np.testing.assert_array_equal(out.cpu().numpy(), want)
     assert moves == want_moves
     np.testing.assert_array_equal(G.alpha_beta_host(data, table, init, iters), want)
+    if W < 3 or H < 3:
+        np.testing.assert_array_equal(want, init)
+        assert moves == 0
 
 
 def test_alpha_beta_refines_bp(oracle_mod):

@@ -84,8 +84,25 @@ def _cases():
     return out
 
 
+# Register tiles built for the chunked kernel: (labels per lane C, lanes per
+# node G), as GINF_SHAPE_LIST in gridinfcuda_amd/csrc/bp_kernels.h.  A plan
+# with an explicit lane count G supports L <= max C * G over the pairs with
+# that G; the quad kernel and the default chunked shapes go up to 256 labels.
+CHUNKED_SHAPES = [(16, 1), (8, 2), (16, 2), (8, 4), (16, 4), (8, 8), (24, 4), (12, 8), (32, 4), (16, 8), (8, 16),
+                  (16, 16)]
+
+
+def expect_unsupported(L, lanes, kernel):
+    if kernel == 2 or lanes == 0:
+        return L > 256
+    return L > max(c * g for c, g in CHUNKED_SHAPES if g == lanes)
+
+
 @pytest.mark.parametrize("H,W,L,lanes,levels,iters,kind,dt,norm,kernel", _cases())
 def test_linear_bit_exact(oracle_mod, H, W, L, lanes, levels, iters, kind, dt, norm, kernel):
+    """Every case runs: the ones whose (lanes, L) pair has no register tile
+    must fail with GINF_ERR_UNSUPPORTED and nothing else; every other case must
+    run and match the oracle."""
     say(f"linear H={H} W={W} L={L} lanes={lanes} levels={levels} iters={iters} {kind} {dt} norm={norm} "
         f"kernel={kernel}")
     rng = np.random.default_rng(zlib.crc32(repr((H, W, L, lanes, levels, iters, kind, dt, norm)).encode()))
@@ -93,13 +110,16 @@ def test_linear_bit_exact(oracle_mod, H, W, L, lanes, levels, iters, kind, dt, n
     data = rng.integers(0, 40, size=(L, H, W)).astype(dtype)
     table = table_for(oracle_mod, kind, L, dtype)
     init = rng.integers(0, L, size=(H, W)).astype(np.int32)
-    try:
-        # quad cases alternate between 16-bit and T storage of the data costs
-        lab, msgs, _ = run_plan(data, table, SM[kind], levels, iters, norm=norm, lanes=lanes, labels_init=init,
-                                dump_level=0, kernel=kernel, compact=bool((H + W + L) & 1) if kernel == 2 else None)
-    except G.GinfError as e:
-        assert e.code == 2, e  # unsupported (lanes, L) pair
-        pytest.skip(str(e))
+    # quad cases alternate between 16-bit and T storage of the data costs
+    compact = bool((H + W + L) & 1) if kernel == 2 else None
+    if expect_unsupported(L, lanes, kernel):
+        with pytest.raises(G.GinfError) as e:
+            run_plan(data, table, SM[kind], levels, iters, norm=norm, lanes=lanes, labels_init=init, dump_level=0,
+                     kernel=kernel, compact=compact)
+        assert e.value.code == 2, e.value
+        return
+    lab, msgs, _ = run_plan(data, table, SM[kind], levels, iters, norm=norm, lanes=lanes, labels_init=init,
+                            dump_level=0, kernel=kernel, compact=compact)
     ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, SM[kind], levels, iters, norm=norm, labels_init=init,
                                           return_state=True)
     np.testing.assert_array_equal(lab, ref_lab)
