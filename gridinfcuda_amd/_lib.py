@@ -22,7 +22,7 @@ KERNEL_AUTO, KERNEL_CHUNKED, KERNEL_QUAD = 0, 1, 2
 # Every symbol include/ginf_bp.h declares.
 EXPORTS = [
     "ginf_bp_plan_create", "ginf_bp_plan_run", "ginf_bp_plan_destroy", "ginf_bp_plan_workspace_bytes",
-    "ginf_hbp_host", "ginf_energy_f64", "ginf_bp_plan_dump_msgs", "ginf_bp_plan_level_dims",
+    "ginf_hbp_host", "ginf_energy_f64", "ginf_energy_f64_noalloc", "ginf_bp_plan_dump_msgs", "ginf_bp_plan_level_dims",
     "ginf_gen_stereo", "ginf_last_error", "ginf_version", "ginf_bp_plan_set_sweep_timing",
     "ginf_bp_plan_sweep_stats", "ginf_bp_plan_updates_per_run", "ginf_bp_plan_dump_data",
     "ginf_batch_create", "ginf_batch_run", "ginf_batch_set_tiles", "ginf_batch_updates_per_run",
@@ -76,6 +76,7 @@ def lib() -> ctypes.CDLL:
         L.ginf_bp_plan_workspace_bytes.argtypes = [VP, P(ctypes.c_size_t)]
         L.ginf_hbp_host.argtypes = [P(BpDesc), VP, VP, VP]
         L.ginf_energy_f64.argtypes = [I, I, I, I, VP, VP, VP, P(ctypes.c_double), VP]
+        L.ginf_energy_f64_noalloc.argtypes = [I, I, I, I, VP, VP, VP, P(ctypes.c_double), VP]
         L.ginf_bp_plan_dump_msgs.argtypes = [VP, I, VP]
         L.ginf_bp_plan_dump_data.argtypes = [VP, I, VP]
         L.ginf_bp_plan_level_dims.argtypes = [VP, I, P(I), P(I)]
@@ -295,11 +296,14 @@ def max_prior_host(data: np.ndarray) -> np.ndarray:
     return out
 
 
-def energy_f64(width, height, num_labels, dtype, data_ptr, table_ptr, labels_ptr, stream=0) -> float:
+def energy_f64(width, height, num_labels, dtype, data_ptr, table_ptr, labels_ptr, stream=0,
+               noalloc: bool = False) -> float:
+    """Labelling energy in FP64 (ginf_energy_f64, or with noalloc the entry
+    point that keeps its partial sums in the library's fixed device array)."""
     out = ctypes.c_double()
-    check(lib().ginf_energy_f64(width, height, num_labels, dtype, ctypes.c_void_p(data_ptr),
-                                ctypes.c_void_p(table_ptr), ctypes.c_void_p(labels_ptr), ctypes.byref(out),
-                                ctypes.c_void_p(stream)))
+    fn = lib().ginf_energy_f64_noalloc if noalloc else lib().ginf_energy_f64
+    check(fn(width, height, num_labels, dtype, ctypes.c_void_p(data_ptr), ctypes.c_void_p(table_ptr),
+             ctypes.c_void_p(labels_ptr), ctypes.byref(out), ctypes.c_void_p(stream)))
     return out.value
 
 

@@ -792,6 +792,67 @@ int ginf_energy_f64(int width, int height, int num_labels, int dtype, const void
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// The energy without allocation (no reference counterpart): the per-block
+// partial sums and the total of k_energy_partial / k_energy_final live in a
+// fixed device array instead of a hipMalloc'd buffer, for callers that cannot
+// allocate (tight serving loops such as the warm start's per-instance
+// fallback decisions, code that must not synchronise the device through
+// hipMalloc/hipFree).  The array's size does not depend on the grid: the
+// partial-sum kernel is a grid-stride loop over kEnergyBlocks blocks.
+namespace {
+
+constexpr int kEnergyBlocks = 1024;
+constexpr int kEnergyScratch = kEnergyBlocks + 1;  // partials, then the total (8,200 bytes)
+static __device__ __align__(256) double scratch[kEnergyScratch];
+
+// Device address of `scratch` on each device, looked up on the first call
+// made with that device current (each GPU holds its own copy of the array).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+int energy_scratch(int dev, double **out) {
+    if (dev < 0 || dev >= kMaxDevices) return set_error(GINF_ERR_UNSUPPORTED, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(GINF_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_dev[dev] = (double *)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return GINF_OK;
+}
+
+}  // namespace
+
+extern "C" int ginf_energy_f64_noalloc(int width, int height, int num_labels, int dtype, const void *data_dev,
+                                       const void *table_dev, const int32_t *labels_dev, double *energy_host,
+                                       void *stream) {
+    if (!data_dev || !table_dev || !labels_dev || !energy_host) return set_error(GINF_ERR_INVALID, "NULL argument");
+    if (width < 1 || height < 1 || num_labels < 1) return set_error(GINF_ERR_INVALID, "bad sizes");
+    if (dtype != GINF_F32 && dtype != GINF_I32) return set_error(GINF_ERR_INVALID, "dtype %d", dtype);
+    if (width < 3 || height < 3) { *energy_host = 0.0; return GINF_OK; }
+    int dev = 0;
+    GINF_HIP_TRY(hipGetDevice(&dev));
+    double *buf = nullptr;
+    if (int rc = energy_scratch(dev, &buf)) return rc;
+    static_assert(kEnergyBlocks + 1 <= kEnergyScratch, "the energy launch needs kEnergyBlocks partials and a total");
+    hipStream_t st = (hipStream_t)stream;
+    // k_energy_partial writes every one of the kEnergyBlocks partials and
+    // k_energy_final overwrites the total, so nothing has to start at zero.
+    hipError_t e = ginf::launch_energy(dtype, data_dev, table_dev, labels_dev, width, height, num_labels, buf,
+                                       kEnergyBlocks, buf + kEnergyBlocks, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(energy_host, buf + kEnergyBlocks, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return set_error(GINF_ERR_HIP, "energy: %s", hipGetErrorString(e));
+    if (*energy_host != *energy_host) return set_error(GINF_ERR_INVALID, "a label is outside [0, %d)", num_labels);
+    return GINF_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Batches: many single-level instances advanced together.  The work list is
 // grouped by label capacity; each group's half-sweeps are persistent launches
 // of the kernel built for that capacity, on a graph branch of their own (the

@@ -113,6 +113,19 @@ int ginf_hbp_host_release(void);
 int ginf_energy_f64(int width, int height, int num_labels, int dtype, const void *data_dev,
                     const void *table_dev, const int32_t *labels_dev, double *energy_host, void *stream);
 
+/* The same energy without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code in tight serving
+ * loops, such as per-instance fallback decisions; code that must not
+ * synchronise the device through hipMalloc/hipFree).  Its per-block partial
+ * sums live in a fixed 8,200-byte device array of the library (1024 partials
+ * and the total; each device holds one copy until the process ends), whose
+ * size does not depend on the grid, so every grid size is accepted.  All
+ * calls on one device share that array: calls that overlap on one device
+ * race and must be serialised by the caller.  Synchronous, like
+ * ginf_energy_f64. */
+int ginf_energy_f64_noalloc(int width, int height, int num_labels, int dtype, const void *data_dev,
+                            const void *table_dev, const int32_t *labels_dev, double *energy_host, void *stream);
+
 /* Test hook: copy the outbox messages of pyramid level `level` from the last
  * run into msgs_host in the oracle layout [l][d][y][x] (Matrix<T>(4,w,h,4,L),
  * src/bp.cc:125).  Synchronous. */
