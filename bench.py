@@ -567,6 +567,8 @@ def main() -> int:
     ap.add_argument("--no-probe", action="store_true", help="skip the HBM streaming probe")
     ap.add_argument("--no-extras", action="store_true",
                     help="default workload: skip the warm-start and batched figures added to the line")
+    ap.add_argument("--pairs", type=int, default=None, choices=[0, 1],
+                    help="fused pairs of half-sweeps: 1 on, 0 off (default: the plan's choice, on up to 32 labels)")
     ap.add_argument("--data32", action="store_true",
                     help="keep 32-bit data costs (no 16-bit storage even when the costs fit)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
@@ -629,6 +631,8 @@ def main() -> int:
     plan = G.Plan(desc, table)
     if args.data32:
         plan.set_compact_data(False)
+    if args.pairs is not None:
+        plan.set_pairs(bool(args.pairs))
     data = torch.from_numpy(gen).cuda()
     del gen
     labels = torch.zeros((H, W), dtype=torch.int32, device="cuda")
@@ -673,13 +677,16 @@ def main() -> int:
     bytes_per_launch = bytes_per_update * sweep_updates / max(1, launches)
     achieved = bytes_per_launch / (avg_launch_ms / 1000.0) / 1e9
     quad = args.kernel != 1
-    kname = ("k_sweep_quad16" if compact else "k_sweep_quad") if quad else "k_sweep_linear"
+    pairs = plan.pairs()
+    kname = ("k_sweep_pair (two half-sweeps per launch)" if pairs else
+             ("k_sweep_quad16" if compact else "k_sweep_quad") if quad else "k_sweep_linear")
 
     # HBM bytes per steady half-sweep launch, measured with rocprofv3 FETCH_SIZE /
     # WRITE_SIZE (scaled by a 1 GiB copy) on this workload; see the profile summary.
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"{'sweep_quad16' if compact else 'sweep_quad'}_1920x1080x128_traffic.json")
-    default_shape = (W, H, L, args.levels, args.iters, args.norm, args.kernel, args.lanes) == (1920, 1080, 128, 5, 10, 0, 0, 0)
+    default_shape = (W, H, L, args.levels, args.iters, args.norm, args.kernel, args.lanes) == (1920, 1080, 128, 5, 10, 0, 0, 0) \
+        and not pairs
     if default_shape and os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f)["traffic_bytes_per_launch"]
@@ -707,7 +714,7 @@ def main() -> int:
                         f"{args.levels} levels x {args.iters} half-sweeps, "
                         f"{'mean' if args.norm == 0 else 'min'} normalisation",
             "width": W, "height": H, "labels": L, "levels": args.levels, "iters_per_level": args.iters,
-            "node_label_updates_per_step": U, "tile": shape,
+            "node_label_updates_per_step": U, "tile": shape, "fused_pairs": pairs,
         },
         "roofline": {
             "bound": "hbm",

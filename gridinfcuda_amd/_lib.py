@@ -32,7 +32,7 @@ EXPORTS = [
     "ginf_stereo_costs", "ginf_batch_set_seeded", "ginf_batch_seed", "ginf_hbp_host_release",
     "ginf_icm", "ginf_max_prior", "ginf_icm_host", "ginf_max_prior_host", "ginf_alpha_beta",
     "ginf_alpha_beta_host", "ginf_stream_probe", "ginf_bp_plan_set_compact_data", "ginf_bp_plan_compact_data",
-    "ginf_batch_set_compact_data",
+    "ginf_batch_set_compact_data", "ginf_bp_plan_set_pairs", "ginf_bp_plan_pairs",
 ]
 
 
@@ -100,6 +100,8 @@ def lib() -> ctypes.CDLL:
         L.ginf_stereo_costs.argtypes = [VP, VP, I, I, I, I, VP, VP]
         L.ginf_stream_probe.argtypes = [I, I, VP, VP, I64, I, VP]
         L.ginf_bp_plan_set_compact_data.argtypes = [VP, I]
+        L.ginf_bp_plan_set_pairs.argtypes = [VP, I]
+        L.ginf_bp_plan_pairs.argtypes = [VP, P(I)]
         L.ginf_bp_plan_compact_data.argtypes = [VP, P(I)]
         L.ginf_last_error.restype = ctypes.c_char_p
         L.ginf_version.restype = ctypes.c_char_p
@@ -187,6 +189,16 @@ class Plan:
 
     def set_compact_data(self, enable: bool) -> None:
         check(lib().ginf_bp_plan_set_compact_data(self._h, 1 if enable else 0))
+
+    def set_pairs(self, enable) -> None:
+        """Fused pairs of half-sweeps (ginf_bp_plan_set_pairs): True, False, or
+        None for the default (on for label capacities 16 and 32)."""
+        check(lib().ginf_bp_plan_set_pairs(self._h, -1 if enable is None else (1 if enable else 0)))
+
+    def pairs(self) -> bool:
+        v = ctypes.c_int()
+        check(lib().ginf_bp_plan_pairs(self._h, ctypes.byref(v)))
+        return bool(v.value)
 
     def compact_data(self) -> bool:
         """Whether the last run stored the data costs as 16-bit integers."""

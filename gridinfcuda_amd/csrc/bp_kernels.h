@@ -73,6 +73,19 @@ hipError_t launch_sweep_quad16(int dtype, const SweepShape &sh, const SweepArgs 
                                hipStream_t st);
 hipError_t launch_sweep_general(int dtype, const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
                                 hipStream_t st);
+// Two half-sweeps i (moving colour aA.color) and i + 1 (aB.color) of one
+// level in one launch, colour A's messages passed through LDS
+// (sweep_pair.inc); quad layout, label capacities 16, 32, 64, 96 and 128.
+// flags: 1 = labels of colour A, 2 = labels of colour B, 4 = write colour A's
+// outboxes to HBM (no later half-sweep of the level rewrites them).
+hipError_t launch_sweep_pair_f32(int lmax, const SweepArgs &aA, const SweepArgs &aB, bool zeroin, int flags,
+                                 hipStream_t st);
+hipError_t launch_sweep_pair_i32(int lmax, const SweepArgs &aA, const SweepArgs &aB, bool zeroin, int flags,
+                                 hipStream_t st);
+// Workgroups (tiles) of one fused launch on a level with half-row whp and h rows.
+int pair_tiles_f32(int lmax, int whp, int h);
+int pair_tiles_i32(int lmax, int whp, int h);
+inline bool pair_capacity(int lmax) { return lmax == 16 || lmax == 32 || lmax == 64 || lmax == 96 || lmax == 128; }
 hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, const StateLayout &ly, const LevelGeom &g,
                            void *dst0, void *dst1, hipStream_t st);
 // data16: store the level data as 16-bit integers; a value without an exact

@@ -95,6 +95,22 @@ bool pick_quad_shape(int L, ginf::SweepShape &sh) {
     return false;
 }
 
+// Whether the quad kernel's 32-bit buffer offsets reach every state array of
+// a plan of this shape: 4 arrays of level 0, 8 of every coarser level (the
+// bound checked at plan creation).
+bool quad_offsets_fit(const ginf_bp_desc *d) {
+    const int L = d->num_labels;
+    const int64_t planes = (L + 3) / 4;
+    int64_t w = d->width, h = d->height;
+    for (int t = 0; t < d->num_levels; ++t) {
+        if (t > 0) { w = (w + 1) / 2; h = (h + 1) / 2; }
+        const int64_t whp = round_up((w + 1) / 2, 16);
+        const int64_t arr = round_up(planes * round_up(whp * h, ginf::kTileSlots) * 4 * 4, 256);
+        if ((t == 0 ? 4 : 8) * arr >= ((int64_t)1 << 32)) return false;
+    }
+    return true;
+}
+
 bool pick_general_shape(int L, int lanes, ginf::SweepShape &sh) {
     if (lanes > 1) return false;
     for (int c : {16, 32, 64, 128})
@@ -126,6 +142,11 @@ struct ginf_bp_plan_s {
     bool compact = true;
     bool compact_last = false;
     int *flag_dev = nullptr;
+    // fused pairs of half-sweeps (ginf_bp_plan_set_pairs) and, per level, the
+    // second set of colour-0 outbox arrays they alternate with
+    int pair_mode = -1;  // -1: fused where measured faster (label capacity <= 32), 0: off, 1: on
+    void *alt_pool = nullptr;
+    std::vector<void *> alt;
     // sweep timing (profiling hook)
     bool timing = false;
     struct Timed { hipEvent_t beg, end; double updates; };
@@ -281,6 +302,39 @@ hipEvent_t take_event(ginf_bp_plan p) {
     return e;
 }
 
+constexpr int kPairMinTiles = 256;
+
+// Whether runs of the plan fuse pairs of half-sweeps (sweep_pair.inc): on
+// request, or by default for the label capacities where fused pairs measured
+// faster than the unfused half-sweeps (16 and 32).
+bool pairs_on(ginf_bp_plan p) {
+    if (!p->shape.quad || !ginf::pair_capacity(p->shape.C)) return false;
+    return p->pair_mode == 1 || (p->pair_mode == -1 && p->shape.C <= 32);
+}
+
+// The second set of colour-0 outbox arrays fused pairs alternate with, per
+// level; zeroed once, since border outboxes are never written (as in the
+// level's own arrays).
+int plan_alloc_pairs(ginf_bp_plan p) {
+    if (!pairs_on(p) || p->alt_pool) return GINF_OK;
+    size_t total = 0;
+    for (int t = 0; t < p->levels; ++t) total += 4 * p->arr_bytes[t];
+    GINF_HIP_TRY(hipMalloc(&p->alt_pool, total));
+    const hipError_t e = hipMemset(p->alt_pool, 0, total);
+    if (e != hipSuccess) {
+        (void)hipFree(p->alt_pool);
+        p->alt_pool = nullptr;
+        return set_error(GINF_ERR_HIP, "pair buffers: %s", hipGetErrorString(e));
+    }
+    p->alt.clear();
+    size_t off = 0;
+    for (int t = 0; t < p->levels; ++t) {
+        p->alt.push_back((char *)p->alt_pool + off);
+        off += 4 * p->arr_bytes[t];
+    }
+    return GINF_OK;
+}
+
 // The HBP driver.  iters[t] half-sweeps run at level t for t <= start;
 // levels above `start` are skipped.  When `seeded`, level `start` begins from
 // the outboxes already in its arrays (written by ginf_bp_plan_seed) instead
@@ -350,14 +404,12 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
             GINF_HIP_TRY(launch_zero_labels(labels_dev, W, H, 0, st));
             if (N == 0) GINF_HIP_TRY(launch_zero_labels(labels_dev, W, H, 1, st));
         }
-        for (int i = 0; i < N; ++i) {
+        auto make_args = [&](int i) {
             SweepArgs a{};
             const int color = (i + 1) & 1;  // (x + y + i) odd (src/bp.cc:150)
             a.data = p->lv[t].data[color];
             for (int k = 0; k < 4; ++k) a.out[k] = p->lv[t].msgs[color][k];
-            bool zeroin = false;
             const int src_level = (fused && i == 0 && !top) ? t + 1 : t;
-            if (fused && i == 0 && top) zeroin = true;
             for (int c = 0; c < 2; ++c)
                 for (int k = 0; k < 4; ++k) a.in[c][k] = p->lv[src_level].msgs[c][k];
             a.src = src_level == t ? kSrcSame : kSrcCoarse;
@@ -391,21 +443,64 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
                 a.in_plane_bytes = a.plane_bytes;
                 a.in_span = (uint32_t)(8 * p->arr_bytes[src_level]);
             }
-            const bool labels = (t == 0) && (i >= N - 2);
+            return a;
+        };
+        // Fused pairs of half-sweeps (quad layout): colour A's messages of
+        // half-sweep i reach colour B at i + 1 through LDS.  A pair's stage A
+        // reads colour B's outboxes of the previous pair at its tile edges
+        // while neighbouring tiles write colour B's new outboxes, so
+        // successive pairs alternate colour B's outboxes (always colour 0)
+        // between the level's arrays and a second set, arranged so that the
+        // level's last pair writes the level's own arrays.  A pair never
+        // reads and writes the same set: where the first pair would read the
+        // level's own colour-B arrays (a seeded top level) and an odd number
+        // of pairs would make it write them too, the first two half-sweeps
+        // run unfused.
+        // A level with fewer tiles than compute units runs unfused: two short
+        // launches are quicker there than one fused launch whose workgroups
+        // each wait for two HBM round trips (128x96x16 plain BP, 10
+        // half-sweeps: 0.044 ms unfused, 0.051 ms fused, 48 tiles).
+        const bool pairs = pairs_on(p) && p->alt_pool && g.w >= 3 && g.h >= 3 &&
+                           ginf::pair_tiles_f32(p->shape.C, g.whp, g.h) >= kPairMinTiles;
+        int npairs = pairs ? N / 2 : 0, first_pair = 0;
+        if (npairs % 2 == 1 && top && seeded) { first_pair = 2; --npairs; }
+        int k = 0;
+        bool prev_alt = false;
+        for (int i = 0; i < N;) {
+            const bool zeroin = fused && i == 0 && top;
+            const bool pair = pairs && i >= first_pair && i + 1 < N;
+            const bool labels_i = (t == 0) && (i >= N - 2);
+            const bool labels_j = pair && (t == 0) && (i + 1 >= N - 2);
+            const SweepArgs a = make_args(i);
             ginf_bp_plan_s::Timed tm{nullptr, nullptr, 0.0};
             // Only the steady-state kernel instance (no zero-input, no labels) is
             // timed, so the figure matches a per-kernel profiler summary.
-            if (p->timing && !zeroin && !labels) {
+            if (p->timing && !zeroin && !labels_i && !labels_j) {
                 tm.beg = take_event(p);
                 tm.end = take_event(p);
-                tm.updates = (double)interior_of_color(g.w, g.h, color) * L;
+                tm.updates = (double)interior_of_color(g.w, g.h, a.color) * L;
+                if (pair) tm.updates += (double)interior_of_color(g.w, g.h, 1 - a.color) * L;
                 if (tm.beg) GINF_HIP_TRY(hipEventRecord(tm.beg, st));
             }
-            GINF_HIP_TRY(launch_sweep(dt, p->shape, a, zeroin, labels, st, d16));
+            if (pair) {
+                SweepArgs aa = a;
+                SweepArgs b = make_args(i + 1);
+                if (k > 0 && prev_alt) aa.in_base = p->alt[t];  // the previous pair's colour-B outboxes
+                const bool dst_alt = (npairs - 1 - k) % 2 == 1;
+                if (dst_alt) b.out_base = p->alt[t];
+                prev_alt = dst_alt;
+                ++k;
+                const int flags = (labels_i ? 1 : 0) | (labels_j ? 2 : 0) | (i + 2 >= N ? 4 : 0);
+                GINF_HIP_TRY(dt == GINF_F32 ? launch_sweep_pair_f32(p->shape.C, aa, b, zeroin, flags, st)
+                                            : launch_sweep_pair_i32(p->shape.C, aa, b, zeroin, flags, st));
+            } else {
+                GINF_HIP_TRY(launch_sweep(dt, p->shape, a, zeroin, labels_i, st, d16));
+            }
             if (tm.beg && tm.end) {
                 GINF_HIP_TRY(hipEventRecord(tm.end, st));
                 p->timed.push_back(tm);
             }
+            i += pair ? 2 : 1;
         }
     }
     return GINF_OK;
@@ -439,9 +534,13 @@ int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, gin
         ok = pick_quad_shape(L, sh);
     } else if (desc->kernel == GINF_KERNEL_CHUNKED || desc->lanes_per_node != 0) {
         ok = pick_linear_shape(L, desc->lanes_per_node, sh);
-    } else if (desc->dtype == GINF_F32 && desc->norm == GINF_NORM_MEAN && L > 16) {
+    } else if ((desc->dtype == GINF_F32 && desc->norm == GINF_NORM_MEAN) || (L <= 32 && quad_offsets_fit(desc))) {
         // FP32 mean normalisation rounds differently under any other order:
-        // keep the reference's sequential per-direction order.
+        // keep the reference's sequential per-direction order.  Up to 32
+        // labels the quad kernel (with fused pairs) is also the faster one for
+        // integer arithmetic (1920x1080x16 HBP: 1.29-1.31 ms against 1.65-1.68
+        // ms chunked; x32: 2.68-2.94 against 3.10-3.12; 128x96x16: 0.044-0.046
+        // against 0.060).
         ok = pick_quad_shape(L, sh);
     } else {
         ok = pick_linear_shape(L, 0, sh);
@@ -530,6 +629,10 @@ int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, gin
         delete p;
         return set_error(GINF_ERR_HIP, "plan set-up failed: %s", hipGetErrorString(e));
     }
+    if (int rc2 = plan_alloc_pairs(p)) {
+        ginf_bp_plan_destroy(p);
+        return rc2;
+    }
     *out = p;
     return GINF_OK;
 }
@@ -540,6 +643,7 @@ int ginf_bp_plan_destroy(ginf_bp_plan plan) {
     for (auto e : plan->spare) hipEventDestroy(e);
     if (plan->pool) hipFree(plan->pool);
     if (plan->flag_dev) hipFree(plan->flag_dev);
+    if (plan->alt_pool) hipFree(plan->alt_pool);
     delete plan;
     return GINF_OK;
 }
@@ -553,6 +657,19 @@ int ginf_bp_plan_workspace_bytes(ginf_bp_plan plan, size_t *bytes) {
 int ginf_bp_plan_set_compact_data(ginf_bp_plan plan, int enable) {
     if (!plan) return set_error(GINF_ERR_INVALID, "NULL plan");
     plan->compact = enable != 0;
+    return GINF_OK;
+}
+
+int ginf_bp_plan_set_pairs(ginf_bp_plan plan, int enable) {
+    if (!plan) return set_error(GINF_ERR_INVALID, "NULL plan");
+    if (enable < -1 || enable > 1) return set_error(GINF_ERR_INVALID, "pairs mode %d (-1, 0 or 1)", enable);
+    plan->pair_mode = enable;
+    return plan_alloc_pairs(plan);
+}
+
+int ginf_bp_plan_pairs(ginf_bp_plan plan, int *enabled) {
+    if (!plan || !enabled) return set_error(GINF_ERR_INVALID, "NULL argument");
+    *enabled = pairs_on(plan) ? 1 : 0;
     return GINF_OK;
 }
 

@@ -48,7 +48,7 @@ enum ginf_norm { GINF_NORM_MEAN = 0, GINF_NORM_MIN = 1 };
  * CHUNKED: G lanes per node split the labels (exact for integer arithmetic).
  * QUAD: four lanes per node, one per direction, each running the reference's
  * sequential scans and normaliser (exact for every type and normalisation).
- * AUTO: QUAD for FP32 mean normalisation with L > 16, else CHUNKED. */
+ * AUTO: QUAD for FP32 mean normalisation and for L <= 32, else CHUNKED. */
 enum ginf_kernel { GINF_KERNEL_AUTO = 0, GINF_KERNEL_CHUNKED = 1, GINF_KERNEL_QUAD = 2 };
 
 typedef struct ginf_bp_desc {
@@ -90,6 +90,19 @@ int ginf_bp_plan_destroy(ginf_bp_plan plan);
  * run kept 16-bit costs (it waits for the device). */
 int ginf_bp_plan_set_compact_data(ginf_bp_plan plan, int enable);
 int ginf_bp_plan_compact_data(ginf_bp_plan plan, int *last_run_compact);
+
+/* Fused pairs of half-sweeps (no reference counterpart).  With the quad
+ * kernel (label capacities 16, 32, 64, 96 and 128), half-sweeps i and i + 1 of
+ * a level run in one launch: the first colour's messages reach the second
+ * colour through LDS, and go to HBM only where no later half-sweep of the
+ * level rewrites them, so an update moves fewer bytes.  Results are
+ * identical to the unfused half-sweeps.  mode 1 = on, 0 = off, -1 = the
+ * default: on for label capacities 16 and 32, where they measured faster.
+ * The plan then holds a second set of its colour-0 outbox arrays (half its
+ * message memory again).  ginf_bp_plan_pairs reports whether runs of the
+ * plan use them. */
+int ginf_bp_plan_set_pairs(ginf_bp_plan plan, int mode);
+int ginf_bp_plan_pairs(ginf_bp_plan plan, int *enabled);
 
 /* Bytes of device workspace the plan holds. */
 int ginf_bp_plan_workspace_bytes(ginf_bp_plan plan, size_t *bytes);

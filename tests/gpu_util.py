@@ -22,9 +22,10 @@ def say(*args):
 
 def run_plan(data: np.ndarray, table: np.ndarray, model: int, levels: int, iters: int, norm: int = 0,
              lanes: int = 0, labels_init: np.ndarray | None = None, dump_level: int | None = None,
-             runs: int = 1, kernel: int = 0, compact: bool | None = None):
+             runs: int = 1, kernel: int = 0, compact: bool | None = None, pairs: bool | None = None):
     """HBP on the GPU via ginf_bp_plan_*; returns (labels, msgs-or-None, plan).
-    compact: None keeps the plan's default (16-bit data costs when they fit)."""
+    compact: None keeps the plan's default (16-bit data costs when they fit);
+    pairs: None keeps the plan default for fused pairs of half-sweeps."""
     torch = torch_cuda()
     data = np.ascontiguousarray(data)
     L, H, W = data.shape
@@ -32,6 +33,8 @@ def run_plan(data: np.ndarray, table: np.ndarray, model: int, levels: int, iters
     plan = G.Plan(desc, np.ascontiguousarray(table, dtype=data.dtype))
     if compact is not None:
         plan.set_compact_data(compact)
+    if pairs is not None:
+        plan.set_pairs(pairs)
     dd = torch.from_numpy(data).cuda()
     init = np.zeros((H, W), np.int32) if labels_init is None else labels_init.astype(np.int32)
     out = None
