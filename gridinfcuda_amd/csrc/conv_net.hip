@@ -89,6 +89,77 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ x,
     }
 }
 
+// The same convolution with every operand of a group of taps loaded before
+// the group's MFMAs: the loop above waits for one memory round trip per
+// (tap, 32-channel step), which bounds a layer at 1/4 resolution (one wave
+// per SIMD, nothing to hide the latency with) to about 13 us.  CS = Cin / 32
+// channel steps; TAPS = taps per group (all nine for CS = 1, one kernel row
+// otherwise, so the operands fit the register file).  The MFMAs run in the
+// same order as above (tap-major, channel steps inside), so the result is
+// identical.
+template <int CS, int TAPS>
+__global__ __launch_bounds__(256) void k_conv3x3_pf(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
+                                                    const float *__restrict__ bias, int N, int H, int W, int Cout,
+                                                    int relu, uint16_t *__restrict__ yb, float *__restrict__ yf) {
+    constexpr int Cin = 32 * CS;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+    if (p0 >= P) return;
+    const int cg = blockIdx.y;
+    const int m = lane & 15, kg = lane >> 4;
+    const int64_t p = p0 + m;
+    const bool valid = p < P;
+    const int n = valid ? (int)(p / ((int64_t)H * W)) : 0;
+    const int rem = valid ? (int)(p - (int64_t)n * H * W) : 0;
+    const int h = rem / W, wc = rem - (rem / W) * W;
+    f4 acc0 = f4{0.f, 0.f, 0.f, 0.f}, acc1 = f4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero = __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
+    const int64_t wstride = (int64_t)16 * 9 * Cin;
+#pragma unroll
+    for (int g0 = 0; g0 < 9; g0 += TAPS) {
+        bf16x8 a[TAPS][CS], b0[TAPS][CS], b1[TAPS][CS];
+#pragma unroll
+        for (int t = 0; t < TAPS; ++t) {
+            const int kh = (g0 + t) / 3, kw = (g0 + t) % 3;
+            const int hh = h + kh - 1, ww = wc + kw - 1;
+            const bool inb = valid && hh >= 0 && hh < H && ww >= 0 && ww < W;
+            // out-of-image taps read a clamped, in-bounds address and are replaced by zero
+            const uint16_t *xp = x + (((int64_t)n * H + (inb ? hh : 0)) * W + (inb ? ww : 0)) * Cin + kg * 8;
+            const uint16_t *wp = w + ((int64_t)(cg * 32 + m) * 9 + kh * 3 + kw) * Cin + kg * 8;
+#pragma unroll
+            for (int c = 0; c < CS; ++c) {
+                a[t][c] = inb ? load8(xp + 32 * c) : zero;
+                b0[t][c] = load8(wp + 32 * c);
+                b1[t][c] = load8(wp + wstride + 32 * c);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TAPS; ++t)
+#pragma unroll
+            for (int c = 0; c < CS; ++c) {
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t][c], b0[t][c], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t][c], b1[t][c], acc1, 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int co = cg * 32 + t * 16 + m;
+        const float bv = bias ? bias[co] : 0.f;
+        const f4 acc = t ? acc1 : acc0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t pp = p0 + kg * 4 + i;
+            if (pp >= P) continue;
+            float v = acc[i] + bv;
+            if (relu) v = v > 0.f ? v : 0.f;
+            if (yb) yb[pp * Cout + co] = f2bf(v);
+            if (yf) yf[pp * Cout + co] = v;
+        }
+    }
+}
+
 // dW[co][kh][kw][ci] = sum_p dz[p][co] * x[p + (kh-1, kw-1)][ci]: one block per
 // (16-channel co tile, 16-channel ci tile, tap); its four waves split the
 // pixels and add their partial sums in LDS (fixed order, deterministic).
@@ -400,8 +471,18 @@ int ginf_conv3x3_fwd(const uint16_t *x, const uint16_t *w, const float *bias, in
         return GINF_ERR_INVALID;
     const int64_t P = (int64_t)N * H * W;
     const dim3 grid((unsigned)((P + 63) / 64), (unsigned)(Cout / 32)), block(256);
-    hipLaunchKernelGGL(k_conv3x3, grid, block, 0, (hipStream_t)stream, x, w, bias, N, H, W, Cin, Cout, relu, y_bf16,
-                       y_f32);
+    hipStream_t st = (hipStream_t)stream;
+#define GINF_CONV_PF(CS, TAPS) \
+    hipLaunchKernelGGL((k_conv3x3_pf<CS, TAPS>), grid, block, 0, st, x, w, bias, N, H, W, Cout, relu, y_bf16, y_f32)
+    switch (Cin / 32) {
+        case 1: GINF_CONV_PF(1, 9); break;
+        case 2: GINF_CONV_PF(2, 3); break;
+        case 3: GINF_CONV_PF(3, 3); break;
+        case 4: GINF_CONV_PF(4, 3); break;
+        default:
+            hipLaunchKernelGGL(k_conv3x3, grid, block, 0, st, x, w, bias, N, H, W, Cin, Cout, relu, y_bf16, y_f32);
+    }
+#undef GINF_CONV_PF
     return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
 }
 

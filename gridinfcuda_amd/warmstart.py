@@ -396,9 +396,11 @@ def _timed(torch, fn, reps=5):
 # (seed level, half-sweeps per level from the seeded level down to level 0);
 # seed level -1 is plain HBP from zero messages on all 5 levels (a baseline
 # with fewer iterations than the reference's 10 per level).
-SCHEDULES = [(0, (4,)), (0, (6,)), (0, (8,)), (0, (10,)), (0, (14,)),
-             (1, (2, 4)), (1, (4, 4)), (1, (4, 6)), (1, (6, 6)), (1, (6, 8)), (1, (10, 10)),
-             (2, (4, 4, 6)), (2, (6, 6, 6)), (2, (6, 6, 8)), (2, (10, 10, 10)),
+SCHEDULES = [(0, (4,)), (0, (6,)), (0, (8,)), (0, (10,)), (0, (12,)), (0, (14,)),
+             (1, (2, 4)), (1, (4, 4)), (1, (4, 6)), (1, (6, 6)), (1, (6, 8)), (1, (0, 10)), (1, (4, 10)),
+             (1, (6, 10)), (1, (10, 10)),
+             (2, (4, 4, 6)), (2, (6, 6, 6)), (2, (6, 6, 8)), (2, (0, 10, 10)), (2, (2, 10, 10)), (2, (4, 10, 10)),
+             (2, (6, 10, 10)), (2, (10, 10, 10)),
              (-1, (4, 4, 4, 4, 4)), (-1, (6, 6, 6, 6, 6)), (-1, (10, 10, 10, 6, 6)), (-1, (10, 10, 10, 10, 8))]
 
 

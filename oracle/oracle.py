@@ -66,6 +66,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.oracle_ab_move.argtypes = [P, P, i, i, i, P, i, i]
     L.oracle_alpha_beta.argtypes = [P, P, i, i, i, P, i, P]
     L.oracle_alpha_beta.restype = i
+    L.oracle_alpha_beta_ref.argtypes = [P, P, i, i, i, P, i, P, i]
+    L.oracle_alpha_beta_ref.restype = i
     L.oracle_max_threads.restype = ctypes.c_int
 
 
@@ -193,6 +195,21 @@ def alpha_beta(data: np.ndarray, table: np.ndarray, initial: np.ndarray, iters: 
     L, H, W = data.shape
     out = np.zeros((H, W), np.int32)
     n = lib().oracle_alpha_beta(_ptr(data), _ptr(table), W, H, L, _ptr(init), iters, _ptr(out))
+    return out, n
+
+
+def alpha_beta_ref(data: np.ndarray, table: np.ndarray, initial: np.ndarray, iters: int, max_loops: int = 1 << 20):
+    """decodeAlphaBeta as the reference runs it (src/graphcut.cc:197-279):
+    one graph state for the whole decode, rebuilt per pair only at the pair's
+    nodes, sequential push-relabel.  Returns (labels, kept moves)."""
+    data = np.ascontiguousarray(data, dtype=np.int32)
+    table = np.ascontiguousarray(table, dtype=np.int32)
+    init = np.ascontiguousarray(initial, dtype=np.int32)
+    L, H, W = data.shape
+    out = np.zeros((H, W), np.int32)
+    n = lib().oracle_alpha_beta_ref(_ptr(data), _ptr(table), W, H, L, _ptr(init), iters, _ptr(out), max_loops)
+    if n < 0:
+        raise RuntimeError("alpha_beta_ref: a push-relabel loop did not end")
     return out, n
 
 
