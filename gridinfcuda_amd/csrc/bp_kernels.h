@@ -49,7 +49,9 @@ inline StateLayout state_layout(const SweepShape &sh, int L) {
     X(32, 4)               \
     X(16, 8)               \
     X(8, 16)               \
-    X(16, 16)
+    X(16, 16)              \
+    X(16, 32)              \
+    X(16, 64)
 
 struct UpsampleArgs {
     const void *coarse[2][kNumDir];

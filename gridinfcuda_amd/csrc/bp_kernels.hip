@@ -443,7 +443,24 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
     const int LP = ly.C * ly.G;
     const int tw = (LP <= GINF_RELAYOUT_TW64_MAX_LP && 32 * ly.G <= 256) ? 64 : 32;
     const size_t lds = (size_t)2 * LP * (tw + 1) * 4;
-    if (lds > 160 * 1024 || (tw / 2) * ly.G > 256) return hipErrorInvalidValue;
+    if (lds > 160 * 1024 || (tw / 2) * ly.G > 256) {
+        // Layouts past the LDS tile (32 or 64 lanes per node, or more than
+        // about 600 labels): per-column relayout of level 0, then level 1
+        // summed from it.  Only T storage reaches here (16-bit data costs are
+        // a quad-layout form).
+        if (data16 || flag || only_if) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)((W + 255) / 256), (unsigned)H), block(256);
+        if (dtype == 0)
+            hipLaunchKernelGGL(k_relayout<float>, grid, block, 0, st, (const float *)src, W, H, L, ly, g0,
+                               (float *)d00, (float *)d01);
+        else
+            hipLaunchKernelGGL(k_relayout<int>, grid, block, 0, st, (const int *)src, W, H, L, ly, g0, (int *)d00,
+                               (int *)d01);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && coarse)
+            e = launch_pyramid(dtype, d00, d01, g0, d10, d11, g1, ly.C, ly.G, st, false, nullptr, nullptr);
+        return e;
+    }
     const dim3 grid((unsigned)((W + tw - 1) / tw), (unsigned)((H + 1) / 2)), block(256);
 #define GINF_RP(TT, SS, TWW)                                                                                     \
     hipLaunchKernelGGL((k_relayout_pyr<TT, SS, TWW>), grid, block, lds, st, (const TT *)src, W, H, L, ly, g0,      \

@@ -70,7 +70,7 @@ bool pick_linear_shape(int L, int lanes, ginf::SweepShape &sh) {
 #undef GINF_PAIR
     };
     if (lanes == 0) {
-        int g = L <= 16 ? 1 : L <= 32 ? 2 : L <= 64 ? 4 : L <= 128 ? 8 : L <= 256 ? 16 : 0;
+        int g = L <= 16 ? 1 : L <= 32 ? 2 : L <= 64 ? 4 : L <= 128 ? 8 : L <= 256 ? 16 : L <= 512 ? 32 : L <= 1024 ? 64 : 0;
         if (g == 0) return false;
         if (L > 64 && L <= 96) { sh = {12, 8, false, false}; return true; }
         sh = {16, g, false, false};
@@ -115,6 +115,8 @@ bool pick_general_shape(int L, int lanes, ginf::SweepShape &sh) {
     if (lanes > 1) return false;
     for (int c : {16, 32, 64, 128})
         if (L <= c) { sh = {c, 1, true, false}; return true; }
+    // one wave per node (k_sweep_general_wave), L planes
+    if (L <= 1024) { sh = {L, 1, true, false}; return true; }
     return false;
 }
 
@@ -263,8 +265,8 @@ int validate_desc(const ginf_bp_desc *d) {
     if (d->iters_per_level < 0) return set_error(GINF_ERR_INVALID, "iters_per_level %d < 0", d->iters_per_level);
     if (d->norm != GINF_NORM_MEAN && d->norm != GINF_NORM_MIN) return set_error(GINF_ERR_INVALID, "norm %d", d->norm);
     if (d->kernel < GINF_KERNEL_AUTO || d->kernel > GINF_KERNEL_QUAD) return set_error(GINF_ERR_INVALID, "kernel %d", d->kernel);
-    if (d->lanes_per_node < 0 || d->lanes_per_node > 16 || (d->lanes_per_node & (d->lanes_per_node - 1)))
-        return set_error(GINF_ERR_INVALID, "lanes_per_node %d (0, 1, 2, 4, 8 or 16)", d->lanes_per_node);
+    if (d->lanes_per_node < 0 || d->lanes_per_node > 64 || (d->lanes_per_node & (d->lanes_per_node - 1)))
+        return set_error(GINF_ERR_INVALID, "lanes_per_node %d (0 or a power of two up to 64)", d->lanes_per_node);
     for (int r : d->reserved)
         if (r != 0) return set_error(GINF_ERR_INVALID, "reserved fields must be zero");
     if ((int64_t)d->width * d->height > (int64_t)1 << 30)
@@ -534,7 +536,8 @@ int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, gin
         ok = pick_quad_shape(L, sh);
     } else if (desc->kernel == GINF_KERNEL_CHUNKED || desc->lanes_per_node != 0) {
         ok = pick_linear_shape(L, desc->lanes_per_node, sh);
-    } else if ((desc->dtype == GINF_F32 && desc->norm == GINF_NORM_MEAN) || (L <= 32 && quad_offsets_fit(desc))) {
+    } else if ((desc->dtype == GINF_F32 && desc->norm == GINF_NORM_MEAN && L <= 256) ||
+               (L <= 32 && quad_offsets_fit(desc))) {
         // FP32 mean normalisation rounds differently under any other order:
         // keep the reference's sequential per-direction order.  Up to 32
         // labels the quad kernel (with fused pairs) is also the faster one for

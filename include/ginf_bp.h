@@ -48,7 +48,10 @@ enum ginf_norm { GINF_NORM_MEAN = 0, GINF_NORM_MIN = 1 };
  * CHUNKED: G lanes per node split the labels (exact for integer arithmetic).
  * QUAD: four lanes per node, one per direction, each running the reference's
  * sequential scans and normaliser (exact for every type and normalisation).
- * AUTO: QUAD for FP32 mean normalisation and for L <= 32, else CHUNKED. */
+ * AUTO: QUAD for FP32 mean normalisation up to 256 labels and for L <= 32,
+ * else CHUNKED (up to 1024 labels).  The O(L^2) models (EXPLICIT,
+ * TRUNC_QUAD) run one lane per node up to 128 labels and one wavefront per
+ * node up to 1024. */
 enum ginf_kernel { GINF_KERNEL_AUTO = 0, GINF_KERNEL_CHUNKED = 1, GINF_KERNEL_QUAD = 2 };
 
 typedef struct ginf_bp_desc {
@@ -60,7 +63,7 @@ typedef struct ginf_bp_desc {
     int num_levels;       /* >= 1; 1 = plain loopy BP (decodeBp) */
     int iters_per_level;  /* >= 0 red-black half-sweeps per level */
     int norm;             /* ginf_norm */
-    int lanes_per_node;   /* chunked kernel: 0 = default; else 1,2,4,8,16 lanes share a node's labels */
+    int lanes_per_node;   /* chunked kernel: 0 = default; else 1, 2, 4, ..., 64 lanes share a node's labels */
     int kernel;           /* ginf_kernel */
     int reserved[6];      /* must be zero */
 } ginf_bp_desc;

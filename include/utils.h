@@ -19,13 +19,15 @@
 #define GINF_ABS(x) ((x) < 0 ? (-(x)) : (x))
 #define GINF_SQ(x) ((x) * (x))
 
-// The reference capped labels at 16 (its GPU kernel kept a 16-entry local
-// array); the MI355X kernels support up to 256 labels for the O(L) models
-// and 128 for the general models.  The macro keeps its old value for
-// source compatibility.
+// The reference capped labels at 16 on its GPU path (a 16-entry local
+// array; its CPU path has no cap); the MI355X kernels support up to 1024
+// labels for every model (bit-exact against the reference CPU path up to
+// 256 labels for FP32 with mean normalisation, at every count for int and
+// for integer-valued FP32 costs with min normalisation).  The macro keeps
+// its old value for source compatibility.
 #define GINF_MAX_NUM_LABELS 16
-#define GINF_MAX_NUM_LABELS_LINEAR 256
-#define GINF_MAX_NUM_LABELS_GENERAL 128
+#define GINF_MAX_NUM_LABELS_LINEAR 1024
+#define GINF_MAX_NUM_LABELS_GENERAL 1024
 
 namespace ginf {
 enum Directions { GINF_DIR_S, GINF_DIR_E, GINF_DIR_N, GINF_DIR_W };
