@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "ginf_bp.h"
 #include "bp_layout.h"
 
@@ -163,8 +165,15 @@ __global__ __launch_bounds__(256) void k_conv3x3_pf(const uint16_t *__restrict__
 // dW[co][kh][kw][ci] = sum_p dz[p][co] * x[p + (kh-1, kw-1)][ci]: one block per
 // (16-channel co tile, 16-channel ci tile, tap); its four waves split the
 // pixels and add their partial sums in LDS (fixed order, deterministic).
+// The pixels are split into gridDim.y chunks of `chunk` pixels: block
+// (tile, chunk) writes its chunk's sum to part[chunk][co][tap][ci] (or
+// straight to dw with one chunk), and k_dw_reduce adds the chunks in index
+// order, so the result does not depend on scheduling.  A layer at 160x120 x
+// 16 spread over (co, ci, tap) tiles alone is 36-72 blocks each walking all
+// 307,200 pixels (105.6 us per layer).
 __global__ __launch_bounds__(256) void k_conv3x3_dw(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dz,
-                                                    int N, int H, int W, int Cin, int Cout, float *__restrict__ dw) {
+                                                    int N, int H, int W, int Cin, int Cout, int64_t chunk,
+                                                    float *__restrict__ dw) {
     __shared__ float red[4][256];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ct = Cout / 16, it = Cin / 16;
@@ -174,25 +183,29 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw(const uint16_t *__restrict__
     const int to = b;
     const int kh = tap / 3, kw = tap % 3;
     const int m = lane & 15, kg = lane >> 4;
-    const int64_t P = (int64_t)N * H * W;
     const int64_t HW = (int64_t)H * W;
+    const int64_t k0 = (int64_t)blockIdx.y * chunk;
+    const int64_t P = k0 + chunk < (int64_t)N * HW ? k0 + chunk : (int64_t)N * HW;  // end of this chunk
     f4 acc = f4{0.f, 0.f, 0.f, 0.f};
-    for (int64_t q0 = (int64_t)wave * 32; q0 < P; q0 += 4 * 32) {
+    for (int64_t q0 = k0 + (int64_t)wave * 32; q0 < P; q0 += 4 * 32) {
         uint16_t av[8], bv[8];
+        // row and column of this lane's first pixel, then stepped (no division per pixel)
+        const int64_t pb = q0 + kg * 8;
+        const int rem0 = (int)(pb % HW);
+        int h0 = rem0 / W, w0 = rem0 - (rem0 / W) * W;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int64_t p = q0 + kg * 8 + j;
+            const int64_t p = pb + j;
             uint16_t a = 0, bb = 0;
             if (p < P) {
                 a = dz[p * Cout + to * 16 + m];
-                const int n = (int)(p / HW);
-                const int rem = (int)(p - n * HW);
-                const int hh = rem / W + kh - 1, ww = rem % W + kw - 1;
+                const int hh = h0 + kh - 1, ww = w0 + kw - 1;
                 if (hh >= 0 && hh < H && ww >= 0 && ww < W)
-                    bb = x[(((int64_t)n * H + hh) * W + ww) * Cin + ti * 16 + m];
+                    bb = x[(p + (int64_t)(kh - 1) * W + (kw - 1)) * Cin + ti * 16 + m];
             }
             av[j] = a;
             bv[j] = bb;
+            if (++w0 == W) { w0 = 0; if (++h0 == H) h0 = 0; }
         }
         bf16x8 A, B;
 #pragma unroll
@@ -209,24 +222,180 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw(const uint16_t *__restrict__
         const int r = threadIdx.x / 16, c = threadIdx.x % 16;  // r: co within tile, c: ci within tile
         const float v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
         const int co = to * 16 + r, ci = ti * 16 + c;
-        dw[((int64_t)co * 9 + tap) * Cin + ci] = v;
+        dw[(int64_t)blockIdx.y * Cout * 9 * Cin + ((int64_t)co * 9 + tap) * Cin + ci] = v;
     }
 }
 
-// db[co] = sum_p dz[p][co] (one block per channel, fixed-order tree).
-__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ dz, int64_t P, int Cout, float *db) {
-    __shared__ float red[256];
-    const int co = blockIdx.x;
-    float acc = 0.f;
-    for (int64_t p = threadIdx.x; p < P; p += 256) acc += bf2f(dz[p * Cout + co]);
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+// The weight gradient with its operands staged through LDS.  The MFMA
+// reduces over K = pixels, so each lane needs eight consecutive pixels of one
+// channel, while NHWC memory holds one pixel's channels contiguously: block
+// (pixel block, 16-channel co tile) walks image patches of 4 rows x 32
+// columns, loads the patch's dz (16 channels) and the x window around it (6
+// rows x 34 columns x Cin) with 16-byte reads, and stores both channel-major
+// in LDS -- x three times, shifted by the kernel column, so that every
+// operand is one aligned 16-byte LDS read.  Wave w accumulates taps w, w + 4,
+// w + 8 for every ci tile over all of its block's patches; pixels outside the
+// image are zero in both operands (the zero padding of the forward pass).
+// The block's sums go to part[block][co][tap][ci] (straight to dw with one
+// block), which k_chunk_reduce adds in block order.  CIT = Cin / 16.
+constexpr int kDwPR = 4, kDwPC = 32, kDwRow = 40;  // patch rows, columns; LDS row pitch (elements)
+
+// CT = 16-channel co tiles per block: 1 (the four waves split the taps: wave
+// w takes taps w, w + 4, w + 8) or 4 (wave w takes co tile w and all nine
+// taps, so the x window staged for a patch serves 64 output channels).
+template <int CIT, int CT>
+__global__ __launch_bounds__(256) void k_conv3x3_dw_lds(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dz,
+                                                        int N, int H, int W, int Cout, float *__restrict__ out) {
+    constexpr int Cin = 16 * CIT;
+    constexpr int WC = kDwPC + 2, WR = kDwPR + 2, C8 = Cin / 8;
+    constexpr int NX = (WR * WC * C8 + 255) / 256;  // x window loads per thread
+    constexpr int NA = CT * 2 * kDwPR * kDwPC / 256;  // dz loads per thread (16-byte halves of 16 channels)
+    constexpr int NT = CT == 1 ? 3 : 9;               // taps per wave
+    __shared__ __attribute__((aligned(16))) uint16_t sA[16 * CT * kDwPR * kDwRow];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[3 * Cin * WR * kDwRow];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kg = lane >> 4;
+    const int co0 = blockIdx.y * 16 * CT;
+    const int ppr = (W + kDwPC - 1) / kDwPC, prow = (H + kDwPR - 1) / kDwPR;
+    const int64_t npatch = (int64_t)N * prow * ppr;
+    const int ntap = CT == 1 ? (wave == 0 ? 3 : 2) : 9;
+    const int tile = CT == 1 ? 0 : wave;  // this wave's co tile within the block
+    f4 acc[NT][CIT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int c = 0; c < CIT; ++c) acc[t][c] = f4{0.f, 0.f, 0.f, 0.f};
+    // A patch's operands are read from HBM into registers one patch ahead of
+    // the MFMAs that use them, so the loads of patch i+1 are in flight while
+    // patch i is reduced.
+    uint4 va[NA], vx[NX];
+    auto fetch = [&](int64_t pi) {
+        const int n = (int)(pi / ((int64_t)prow * ppr));
+        const int rem = (int)(pi - (int64_t)n * prow * ppr);
+        const int r0 = (rem / ppr) * kDwPR, c0 = (rem % ppr) * kDwPC;
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {  // dz patch: 128 pixels x 16*CT channels in 16-byte reads
+            const int t = threadIdx.x + 256 * u, px = t / (2 * CT), h8 = t % (2 * CT);
+            const int hh = r0 + px / kDwPC, ww = c0 + px % kDwPC;
+            va[u] = uint4{0u, 0u, 0u, 0u};
+            if (hh < H && ww < W)
+                va[u] = *reinterpret_cast<const uint4 *>(dz + (((int64_t)n * H + hh) * W + ww) * Cout + co0 + 8 * h8);
+        }
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {  // x window: rows r0-1 .. r0+4, columns c0-1 .. c0+32
+            const int t = threadIdx.x + 256 * u;
+            const int ci8 = t % C8, wcr = t / C8;
+            const int wc = wcr % WC, rr = wcr / WC;
+            const int hh = r0 - 1 + rr, ww = c0 - 1 + wc;
+            vx[u] = uint4{0u, 0u, 0u, 0u};
+            if (t < WR * WC * C8 && hh >= 0 && hh < H && ww >= 0 && ww < W)
+                vx[u] = *reinterpret_cast<const uint4 *>(x + (((int64_t)n * H + hh) * W + ww) * Cin + 8 * ci8);
+        }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int t = threadIdx.x + 256 * u, px = t / (2 * CT), h8 = t % (2 * CT);
+            const int r = px / kDwPC, c = px % kDwPC;
+            const uint16_t *e = reinterpret_cast<const uint16_t *>(&va[u]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sA[((8 * h8 + k) * kDwPR + r) * kDwRow + c] = e[k];
+        }
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            const int t = threadIdx.x + 256 * u;
+            if (t >= WR * WC * C8) break;
+            const int ci8 = t % C8, wcr = t / C8;
+            const int wc = wcr % WC, rr = wcr / WC;
+            const uint16_t *e = reinterpret_cast<const uint16_t *>(&vx[u]);
+#pragma unroll
+            for (int kw = 0; kw < 3; ++kw) {
+                const int cc = wc - kw;
+                if (cc < 0 || cc >= kDwPC) continue;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) sB[((kw * Cin + 8 * ci8 + k) * WR + rr) * kDwRow + cc] = e[k];
+            }
+        }
+    };
+    int64_t pi = blockIdx.x;
+    if (pi < npatch) fetch(pi);
+    for (; pi < npatch; pi += gridDim.x) {
+        stage();
+        __syncthreads();
+        if (pi + gridDim.x < npatch) fetch(pi + gridDim.x);
+#pragma unroll
+        for (int r = 0; r < kDwPR; ++r) {
+            const bf16x8 A = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(
+                                                            &sA[((16 * tile + m) * kDwPR + r) * kDwRow + 8 * kg]));
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                if (t >= ntap) break;
+                const int tap = CT == 1 ? wave + 4 * t : t, kh = tap / 3, kw = tap % 3;
+#pragma unroll
+                for (int c = 0; c < CIT; ++c) {
+                    const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(
+                        &sB[((kw * Cin + 16 * c + m) * WR + r + kh) * kDwRow + 8 * kg]));
+                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t][c], 0, 0, 0);
+                }
+            }
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[co] = red[0];
+    const int64_t nout = (int64_t)Cout * 9 * Cin;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        if (t >= ntap) break;
+        const int tap = CT == 1 ? wave + 4 * t : t;
+#pragma unroll
+        for (int c = 0; c < CIT; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int co = co0 + 16 * tile + 4 * kg + i;
+                out[(int64_t)blockIdx.x * nout + ((int64_t)co * 9 + tap) * Cin + 16 * c + m] = acc[t][c][i];
+            }
+    }
 }
+
+// out[o] = sum over chunks c = 0, 1, ... of part[c][o], in that order.
+__global__ void k_chunk_reduce(const float *__restrict__ part, int nchunk, int64_t n, float *__restrict__ out) {
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (int64_t)gridDim.x * blockDim.x) {
+        float v = 0.f;
+        int c = 0;
+        for (; c + 8 <= nchunk; c += 8) {  // eight loads in flight, added in order
+            float t[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) t[k] = part[(int64_t)(c + k) * n + o];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v += t[k];
+        }
+        for (; c < nchunk; ++c) v += part[(int64_t)c * n + o];
+        out[o] = v;
+    }
+}
+
+// Bias gradient partials: block (channel group, pixel chunk) sums dz over its
+// chunk for CW consecutive channels, 256 / CW pixel rows at a time (each row
+// read as CW * 2 contiguous bytes); the rows' partial sums are added in a
+// fixed order.  part[chunk][co].
+__global__ __launch_bounds__(256) void k_bias_grad_part(const uint16_t *__restrict__ dz, int64_t P, int Cout, int CW,
+                                                        int64_t chunk, float *__restrict__ part) {
+    __shared__ float red[256];
+    const int rows = 256 / CW;
+    const int c = threadIdx.x % CW, r = threadIdx.x / CW;
+    const int co = blockIdx.x * CW + c;
+    const int64_t k0 = (int64_t)blockIdx.y * chunk;
+    const int64_t k1 = k0 + chunk < P ? k0 + chunk : P;
+    float acc = 0.f;
+    for (int64_t p = k0 + r; p < k1; p += rows) acc += bf2f(dz[p * Cout + co]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (r == 0) {
+        float v = red[c];
+        for (int k = 1; k < rows; ++k) v += red[k * CW + c];
+        part[(int64_t)blockIdx.y * Cout + co] = v;
+    }
+}
+
 
 // dz = bf16(dy * (y > 0)) for a ReLU layer (y its output), bf16(dy) otherwise.
 __global__ void k_relu_bwd(const float *__restrict__ dy, const uint16_t *__restrict__ y, int64_t n, uint16_t *dz) {
@@ -491,8 +660,55 @@ int ginf_conv3x3_bwd_weight(const uint16_t *x, const uint16_t *dz, int N, int H,
     if (!x || !dz || !dw || Cin % 16 || Cout % 16) return GINF_ERR_INVALID;
     hipStream_t st = (hipStream_t)stream;
     const unsigned blocks = (unsigned)((Cout / 16) * (Cin / 16) * 9);
-    hipLaunchKernelGGL(k_conv3x3_dw, dim3(blocks), dim3(256), 0, st, x, dz, N, H, W, Cin, Cout, dw);
-    if (db) hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)Cout), dim3(256), 0, st, dz, (int64_t)N * H * W, Cout, db);
+    const int64_t P = (int64_t)N * H * W, nout = (int64_t)Cout * 9 * Cin;
+    // about 32 chunks of whole 128-pixel rounds (one round = the block's four waves x 32 pixels)
+    int64_t chunk = ((P + 31) / 32 + 127) / 128 * 128;
+    if (chunk < 128) chunk = 128;
+    const int nchunk = (int)((P + chunk - 1) / chunk);
+    const int cw = Cout % 64 == 0 ? 64 : (Cout % 32 == 0 ? 32 : 16);
+    // the bias gradient is a much smaller sum per block: more, shorter chunks
+    const int64_t bchunk = std::max<int64_t>(64, (P + 255) / 256);
+    const int nbchunk = (int)((P + bchunk - 1) / bchunk);
+    float *part = nullptr;
+    // weight partials for up to max(nchunk, 512) blocks, then the bias partials
+    const int64_t wparts = std::max<int64_t>(nchunk, 512);
+    const size_t part_n = (size_t)wparts * (size_t)nout + (size_t)nbchunk * (size_t)Cout;
+    if (hipMallocAsync((void **)&part, part_n * sizeof(float), st) != hipSuccess) return GINF_ERR_NOMEM;
+    const int cit = Cin / 16;
+    if (cit == 2 || cit == 4 || cit == 6) {
+        // LDS-staged kernel: one pixel block per ~CU per co tile, every block
+        // walking its share of the 4x32 patches
+        const int64_t npatch = (int64_t)N * ((H + kDwPR - 1) / kDwPR) * ((W + kDwPC - 1) / kDwPC);
+        // four co tiles per block only where the nine-tap accumulators fit in
+        // registers (CIT 2: 72 of them); wider inputs keep one tile per block
+        const int ct = (Cout % 64 == 0 && cit == 2) ? 4 : 1;
+        const int ngroups = Cout / (16 * ct);
+        const int npb = (int)std::max<int64_t>(1, std::min<int64_t>(npatch, std::max(8, 512 / ngroups)));
+        const dim3 grid((unsigned)npb, (unsigned)ngroups);
+        float *pw = npb > 1 ? part : dw;
+#define GINF_DWL(CI, CT) hipLaunchKernelGGL((k_conv3x3_dw_lds<CI, CT>), grid, dim3(256), 0, st, x, dz, N, H, W, Cout, pw)
+        if (ct == 4) GINF_DWL(2, 4);
+        else { if (cit == 2) GINF_DWL(2, 1); if (cit == 4) GINF_DWL(4, 1); if (cit == 6) GINF_DWL(6, 1); }
+#undef GINF_DWL
+        if (npb > 1)
+            hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, st, part, npb, nout,
+                               dw);
+    } else {
+        float *pw = nchunk > 1 ? part : dw;
+        hipLaunchKernelGGL(k_conv3x3_dw, dim3(blocks, (unsigned)nchunk), dim3(256), 0, st, x, dz, N, H, W, Cin, Cout,
+                           chunk, pw);
+        if (nchunk > 1)
+            hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, st, part, nchunk,
+                               nout, dw);
+    }
+    if (db) {
+        float *pb = part + (size_t)wparts * nout;
+        hipLaunchKernelGGL(k_bias_grad_part, dim3((unsigned)(Cout / cw), (unsigned)nbchunk), dim3(256), 0, st, dz, P,
+                           Cout, cw, bchunk, pb);
+        hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)((Cout + 255) / 256)), dim3(256), 0, st, pb, nbchunk,
+                           (int64_t)Cout, db);
+    }
+    if (hipFreeAsync(part, st) != hipSuccess) return GINF_ERR_HIP;
     return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
 }
 

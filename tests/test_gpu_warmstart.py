@@ -27,7 +27,8 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("N,H,W,Ci,Co", [(2, 7, 9, 32, 32), (1, 5, 33, 64, 32), (3, 4, 6, 32, 256), (1, 6, 7, 96, 384),
-                                         (2, 3, 17, 32, 64), (1, 9, 5, 256, 32)])
+                                         (2, 3, 17, 32, 64), (1, 9, 5, 256, 32), (4, 21, 45, 32, 256),
+                                         (2, 37, 70, 64, 32)])
 def test_conv_forward_and_gradients(N, H, W, Ci, Co):
     torch = torch_cuda()
     say(f"conv N={N} H={H} W={W} Ci={Ci} Co={Co}")
