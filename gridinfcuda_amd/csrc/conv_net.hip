@@ -228,43 +228,61 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw(const uint16_t *__restrict__
 
 // The weight gradient with its operands staged through LDS.  The MFMA
 // reduces over K = pixels, so each lane needs eight consecutive pixels of one
-// channel, while NHWC memory holds one pixel's channels contiguously: block
-// (pixel block, 16-channel co tile) walks image patches of 4 rows x 32
-// columns, loads the patch's dz (16 channels) and the x window around it (6
-// rows x 34 columns x Cin) with 16-byte reads, and stores both channel-major
-// in LDS -- x three times, shifted by the kernel column, so that every
-// operand is one aligned 16-byte LDS read.  Wave w accumulates taps w, w + 4,
-// w + 8 for every ci tile over all of its block's patches; pixels outside the
-// image are zero in both operands (the zero padding of the forward pass).
-// The block's sums go to part[block][co][tap][ci] (straight to dw with one
-// block), which k_chunk_reduce adds in block order.  CIT = Cin / 16.
-constexpr int kDwPR = 4, kDwPC = 32, kDwRow = 40;  // patch rows, columns; LDS row pitch (elements)
+// channel, while NHWC memory holds one pixel's channels contiguously.  Block
+// (pixel block, co group of 16*CT channels) walks image patches of 4 rows x
+// 32 columns: it loads the patch's dz and the x window around it (6 rows x 34
+// columns x Cin) with 16-byte reads and stores both as they lie in memory,
+// one pixel per LDS row, with one 16-byte LDS write per load.  The operands
+// are read back transposed by ds_read_b64_tr_b16: in each group of 16 lanes,
+// lane 4r + q names 4 channels of pixel r (r = 0..3) and lane j receives
+// channel j of those 4 pixels, so two reads give a lane its 8 pixels of one
+// channel.  A kernel tap is only an address offset into the x window.  The
+// (tap, ci tile) pairs are dealt round-robin to the four waves and each wave
+// does them for all CT co tiles, so every staged operand feeds several
+// MFMAs.  Pixels outside the image are zero in both operands (the zero
+// padding of the forward pass).  The block's sums go to
+// part[block][co][tap][ci] (straight to dw with one block), which
+// k_chunk_reduce adds in block order.  CIT = Cin / 16.
+constexpr int kDwPR = 4, kDwPC = 32;  // patch rows, columns
+constexpr int kDwBlocks = 256;          // pixel blocks x co groups per launch (about one per CU)
 
-// CT = 16-channel co tiles per block: 1 (the four waves split the taps: wave
-// w takes taps w, w + 4, w + 8) or 4 (wave w takes co tile w and all nine
-// taps, so the x window staged for a patch serves 64 output channels).
+// LDS pitch (elements) of a pixel row of ch channels: the 16 lanes of a
+// transposed read touch 4 rows x 32 bytes, which fall in distinct banks when
+// the pitch in dwords is 8 or 24 modulo 32.
+__host__ __device__ constexpr int dw_pitch(int ch) { return (ch / 2) % 16 == 0 ? ch + 16 : ch; }
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) s16x4 *lds_s16x4;
+
+// two transposed reads: channel (lane % 16) of pixels p .. p+3 and p+4 .. p+7,
+// where p's row sits at `row` and the next pixels follow at `pitch` elements
+__device__ __forceinline__ bf16x8 tr_read8(const uint16_t *row, int pitch) {
+    const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4)row);
+    const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4)(row + 4 * pitch));
+    return __builtin_bit_cast(bf16x8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
+
 template <int CIT, int CT>
-__global__ __launch_bounds__(256) void k_conv3x3_dw_lds(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dz,
-                                                        int N, int H, int W, int Cout, float *__restrict__ out) {
-    constexpr int Cin = 16 * CIT;
-    constexpr int WC = kDwPC + 2, WR = kDwPR + 2, C8 = Cin / 8;
-    constexpr int NX = (WR * WC * C8 + 255) / 256;  // x window loads per thread
-    constexpr int NA = CT * 2 * kDwPR * kDwPC / 256;  // dz loads per thread (16-byte halves of 16 channels)
-    constexpr int NT = CT == 1 ? 3 : 9;               // taps per wave
-    __shared__ __attribute__((aligned(16))) uint16_t sA[16 * CT * kDwPR * kDwRow];
-    __shared__ __attribute__((aligned(16))) uint16_t sB[3 * Cin * WR * kDwRow];
+__global__ __launch_bounds__(256) void k_conv3x3_dw_tr(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dz,
+                                                       int N, int H, int W, int Cout, float *__restrict__ out) {
+    constexpr int Cin = 16 * CIT, CO = 16 * CT;
+    constexpr int WC = kDwPC + 2, WR = kDwPR + 2, C8 = Cin / 8, O8 = CO / 8;
+    constexpr int XP = dw_pitch(Cin), AP = dw_pitch(CO);
+    constexpr int NX = (WR * WC * C8 + 255) / 256;          // x window loads per thread
+    constexpr int NA = (kDwPR * kDwPC * O8 + 255) / 256;    // dz loads per thread
+    constexpr int NQ = 9 * CIT, QW = (NQ + 3) / 4;          // (tap, ci tile) pairs: all, per wave
+    __shared__ __attribute__((aligned(16))) uint16_t sA[kDwPR * kDwPC * AP];
+    __shared__ __attribute__((aligned(16))) uint16_t sX[WR * WC * XP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m = lane & 15, kg = lane >> 4;
-    const int co0 = blockIdx.y * 16 * CT;
+    const int g = lane >> 4, rq = (lane >> 2) & 3, qq = lane & 3;
+    const int co0 = blockIdx.y * CO;
     const int ppr = (W + kDwPC - 1) / kDwPC, prow = (H + kDwPR - 1) / kDwPR;
     const int64_t npatch = (int64_t)N * prow * ppr;
-    const int ntap = CT == 1 ? (wave == 0 ? 3 : 2) : 9;
-    const int tile = CT == 1 ? 0 : wave;  // this wave's co tile within the block
-    f4 acc[NT][CIT];
+    f4 acc[CT][QW];
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+    for (int c = 0; c < CT; ++c)
 #pragma unroll
-        for (int c = 0; c < CIT; ++c) acc[t][c] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < QW; ++t) acc[c][t] = f4{0.f, 0.f, 0.f, 0.f};
     // A patch's operands are read from HBM into registers one patch ahead of
     // the MFMAs that use them, so the loads of patch i+1 are in flight while
     // patch i is reduced.
@@ -274,11 +292,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw_lds(const uint16_t *__restri
         const int rem = (int)(pi - (int64_t)n * prow * ppr);
         const int r0 = (rem / ppr) * kDwPR, c0 = (rem % ppr) * kDwPC;
 #pragma unroll
-        for (int u = 0; u < NA; ++u) {  // dz patch: 128 pixels x 16*CT channels in 16-byte reads
-            const int t = threadIdx.x + 256 * u, px = t / (2 * CT), h8 = t % (2 * CT);
+        for (int u = 0; u < NA; ++u) {  // dz patch: 128 pixels x CO channels
+            const int t = threadIdx.x + 256 * u, px = t / O8, h8 = t % O8;
             const int hh = r0 + px / kDwPC, ww = c0 + px % kDwPC;
             va[u] = uint4{0u, 0u, 0u, 0u};
-            if (hh < H && ww < W)
+            if (t < kDwPR * kDwPC * O8 && hh < H && ww < W)
                 va[u] = *reinterpret_cast<const uint4 *>(dz + (((int64_t)n * H + hh) * W + ww) * Cout + co0 + 8 * h8);
         }
 #pragma unroll
@@ -295,26 +313,13 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw_lds(const uint16_t *__restri
     auto stage = [&]() {
 #pragma unroll
         for (int u = 0; u < NA; ++u) {
-            const int t = threadIdx.x + 256 * u, px = t / (2 * CT), h8 = t % (2 * CT);
-            const int r = px / kDwPC, c = px % kDwPC;
-            const uint16_t *e = reinterpret_cast<const uint16_t *>(&va[u]);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) sA[((8 * h8 + k) * kDwPR + r) * kDwRow + c] = e[k];
+            const int t = threadIdx.x + 256 * u;
+            if (t < kDwPR * kDwPC * O8) *reinterpret_cast<uint4 *>(&sA[(t / O8) * AP + 8 * (t % O8)]) = va[u];
         }
 #pragma unroll
         for (int u = 0; u < NX; ++u) {
             const int t = threadIdx.x + 256 * u;
-            if (t >= WR * WC * C8) break;
-            const int ci8 = t % C8, wcr = t / C8;
-            const int wc = wcr % WC, rr = wcr / WC;
-            const uint16_t *e = reinterpret_cast<const uint16_t *>(&vx[u]);
-#pragma unroll
-            for (int kw = 0; kw < 3; ++kw) {
-                const int cc = wc - kw;
-                if (cc < 0 || cc >= kDwPC) continue;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) sB[((kw * Cin + 8 * ci8 + k) * WR + rr) * kDwRow + cc] = e[k];
-            }
+            if (t < WR * WC * C8) *reinterpret_cast<uint4 *>(&sX[(t / C8) * XP + 8 * (t % C8)]) = vx[u];
         }
     };
     int64_t pi = blockIdx.x;
@@ -325,33 +330,33 @@ __global__ __launch_bounds__(256) void k_conv3x3_dw_lds(const uint16_t *__restri
         if (pi + gridDim.x < npatch) fetch(pi + gridDim.x);
 #pragma unroll
         for (int r = 0; r < kDwPR; ++r) {
-            const bf16x8 A = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(
-                                                            &sA[((16 * tile + m) * kDwPR + r) * kDwRow + 8 * kg]));
+            bf16x8 A[CT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                if (t >= ntap) break;
-                const int tap = CT == 1 ? wave + 4 * t : t, kh = tap / 3, kw = tap % 3;
+            for (int c = 0; c < CT; ++c) A[c] = tr_read8(&sA[(r * kDwPC + 8 * g + rq) * AP + 16 * c + 4 * qq], AP);
 #pragma unroll
-                for (int c = 0; c < CIT; ++c) {
-                    const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(
-                        &sB[((kw * Cin + 16 * c + m) * WR + r + kh) * kDwRow + 8 * kg]));
-                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t][c], 0, 0, 0);
-                }
+            for (int t = 0; t < QW; ++t) {
+                const int q = wave + 4 * t;
+                if (q >= NQ) break;
+                const int tap = q / CIT, ct = q % CIT, kh = tap / 3, kw = tap % 3;
+                const bf16x8 B = tr_read8(&sX[((r + kh) * WC + 8 * g + rq + kw) * XP + 16 * ct + 4 * qq], XP);
+#pragma unroll
+                for (int c = 0; c < CT; ++c) acc[c][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[c], B, acc[c][t], 0, 0, 0);
             }
         }
         __syncthreads();
     }
     const int64_t nout = (int64_t)Cout * 9 * Cin;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        if (t >= ntap) break;
-        const int tap = CT == 1 ? wave + 4 * t : t;
+    for (int t = 0; t < QW; ++t) {
+        const int q = wave + 4 * t;
+        if (q >= NQ) break;
+        const int tap = q / CIT, ct = q % CIT;
 #pragma unroll
-        for (int c = 0; c < CIT; ++c)
+        for (int c = 0; c < CT; ++c)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int co = co0 + 16 * tile + 4 * kg + i;
-                out[(int64_t)blockIdx.x * nout + ((int64_t)co * 9 + tap) * Cin + 16 * c + m] = acc[t][c][i];
+                const int co = co0 + 16 * c + 4 * g + i;
+                out[(int64_t)blockIdx.x * nout + ((int64_t)co * 9 + tap) * Cin + 16 * ct + (lane & 15)] = acc[c][t][i];
             }
     }
 }
@@ -670,26 +675,29 @@ int ginf_conv3x3_bwd_weight(const uint16_t *x, const uint16_t *dz, int N, int H,
     const int64_t bchunk = std::max<int64_t>(64, (P + 255) / 256);
     const int nbchunk = (int)((P + bchunk - 1) / bchunk);
     float *part = nullptr;
-    // weight partials for up to max(nchunk, 512) blocks, then the bias partials
-    const int64_t wparts = std::max<int64_t>(nchunk, 512);
+    // weight partials for up to max(nchunk, kDwBlocks) blocks, then the bias partials
+    const int64_t wparts = std::max<int64_t>(nchunk, kDwBlocks);
     const size_t part_n = (size_t)wparts * (size_t)nout + (size_t)nbchunk * (size_t)Cout;
     if (hipMallocAsync((void **)&part, part_n * sizeof(float), st) != hipSuccess) return GINF_ERR_NOMEM;
     const int cit = Cin / 16;
-    if (cit == 2 || cit == 4 || cit == 6) {
-        // LDS-staged kernel: one pixel block per ~CU per co tile, every block
-        // walking its share of the 4x32 patches
+    // co tiles per block: the most (of 4, 2, 1) dividing Cout / 16 whose
+    // accumulators stay within 80 registers per lane
+    int ct = 1;
+    for (int c : {4, 2})
+        if ((Cout / 16) % c == 0 && c * ((9 * cit + 3) / 4) <= 20) { ct = c; break; }
+    if (cit == 1 || cit == 2 || cit == 3 || cit == 4 || cit == 6 || cit == 8) {
+        // one pixel block per CU and co group, every block walking its share
+        // of the 4x32 patches
         const int64_t npatch = (int64_t)N * ((H + kDwPR - 1) / kDwPR) * ((W + kDwPC - 1) / kDwPC);
-        // four co tiles per block only where the nine-tap accumulators fit in
-        // registers (CIT 2: 72 of them); wider inputs keep one tile per block
-        const int ct = (Cout % 64 == 0 && cit == 2) ? 4 : 1;
         const int ngroups = Cout / (16 * ct);
-        const int npb = (int)std::max<int64_t>(1, std::min<int64_t>(npatch, std::max(8, 512 / ngroups)));
+        const int npb = (int)std::max<int64_t>(1, std::min<int64_t>(npatch, std::max(8, kDwBlocks / ngroups)));
         const dim3 grid((unsigned)npb, (unsigned)ngroups);
         float *pw = npb > 1 ? part : dw;
-#define GINF_DWL(CI, CT) hipLaunchKernelGGL((k_conv3x3_dw_lds<CI, CT>), grid, dim3(256), 0, st, x, dz, N, H, W, Cout, pw)
-        if (ct == 4) GINF_DWL(2, 4);
-        else { if (cit == 2) GINF_DWL(2, 1); if (cit == 4) GINF_DWL(4, 1); if (cit == 6) GINF_DWL(6, 1); }
-#undef GINF_DWL
+#define GINF_DWT(CI, CT) \
+    if (cit == CI && ct == CT) hipLaunchKernelGGL((k_conv3x3_dw_tr<CI, CT>), grid, dim3(256), 0, st, x, dz, N, H, W, Cout, pw)
+        GINF_DWT(1, 4); GINF_DWT(1, 2); GINF_DWT(1, 1); GINF_DWT(2, 4); GINF_DWT(2, 2); GINF_DWT(2, 1);
+        GINF_DWT(3, 2); GINF_DWT(3, 1); GINF_DWT(4, 2); GINF_DWT(4, 1); GINF_DWT(6, 1); GINF_DWT(8, 1);
+#undef GINF_DWT
         if (npb > 1)
             hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, st, part, npb, nout,
                                dw);

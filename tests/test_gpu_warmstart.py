@@ -67,6 +67,30 @@ def test_conv_forward_and_gradients(N, H, W, Ci, Co):
         assert rel(dx.cpu().numpy(), rdx) < 1e-5
 
 
+@pytest.mark.parametrize("N,H,W,Ci,Co", [(2, 11, 40, 16, 64), (1, 5, 9, 16, 32), (1, 4, 8, 16, 16), (2, 5, 33, 32, 48),
+                                         (1, 9, 35, 48, 32), (1, 6, 33, 48, 16), (1, 7, 50, 64, 16),
+                                         (1, 6, 37, 128, 16), (1, 5, 20, 96, 48), (1, 3, 5, 80, 32)])
+def test_conv_weight_gradient_shapes(N, H, W, Ci, Co):
+    """The weight and bias gradients for every (input tiles, co tiles per
+    block) instance of the LDS-staged kernel (16..128 input channels, 16..64
+    output channels per block) and the per-block fallback (80 inputs)."""
+    torch = torch_cuda()
+    say(f"conv weight gradient N={N} H={H} W={W} Ci={Ci} Co={Co}")
+    rng = np.random.default_rng(Ci * 5 + Co * 3 + W)
+    x = C.bf16_round(rng.standard_normal((N, H, W, Ci)))
+    w = C.bf16_round(rng.standard_normal((Co, 3, 3, Ci)) * 0.2)
+    dz = C.bf16_round(rng.standard_normal((N, H, W, Co)))
+    xd, dzd = dev_bf16(torch, x), dev_bf16(torch, dz)
+    dw = torch.empty((Co, 3, 3, Ci), dtype=torch.float32, device="cuda")
+    db = torch.empty(Co, dtype=torch.float32, device="cuda")
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    G.check(G.lib().ginf_conv3x3_bwd_weight(vp(xd), vp(dzd), N, H, W, Ci, Co, vp(dw), vp(db), s))
+    torch.cuda.synchronize()
+    _, rdw, rdb = C.conv3x3_bwd(x, w, dz)
+    assert rel(dw.cpu().numpy(), rdw) < 1e-5
+    assert rel(db.cpu().numpy(), rdb) < 1e-5
+
+
 def test_network_gradients_match_oracle():
     """Six-layer net (L=32 -> 32 x5 -> 128) forward + backward on the GPU vs the
     oracle with the same bf16 rounding points."""
