@@ -32,7 +32,7 @@ EXPORTS = [
     "ginf_stereo_costs", "ginf_batch_set_seeded", "ginf_batch_seed", "ginf_hbp_host_release",
     "ginf_icm", "ginf_max_prior", "ginf_icm_host", "ginf_max_prior_host", "ginf_alpha_beta",
     "ginf_alpha_beta_host", "ginf_stream_probe", "ginf_bp_plan_set_compact_data", "ginf_bp_plan_compact_data",
-    "ginf_batch_set_compact_data", "ginf_bp_plan_set_pairs", "ginf_bp_plan_pairs",
+    "ginf_batch_set_compact_data", "ginf_bp_plan_set_pairs", "ginf_bp_plan_pairs", "ginf_convnet_fwd",
 ]
 
 
@@ -89,6 +89,7 @@ def lib() -> ctypes.CDLL:
         L.ginf_bp_plan_seed.argtypes = [VP, I, VP, I, I, I, F, VP]
         L.ginf_bp_plan_block_mean_msgs.argtypes = [VP, I, I, F, VP, VP]
         L.ginf_conv3x3_fwd.argtypes = [VP, VP, VP, I, I, I, I, I, I, VP, VP, VP]
+        L.ginf_convnet_fwd.argtypes = [I, VP, VP, VP, VP, I, I, I, VP, VP, VP, VP]
         L.ginf_conv3x3_bwd_weight.argtypes = [VP, VP, I, I, I, I, I, VP, VP, VP]
         L.ginf_conv3x3_flip_weights.argtypes = [VP, I, I, VP, VP]
         L.ginf_relu_bwd.argtypes = [VP, VP, I64, VP, VP]

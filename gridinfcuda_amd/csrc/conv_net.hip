@@ -506,6 +506,45 @@ __global__ void k_minpool(const T *__restrict__ d, int W, int H, int L, int f, f
     }
 }
 
+// The same pooling for 4x4 blocks of whole 16-byte rows and L % 8 == 0: one
+// thread per (cell, group of 8 labels) with X fastest, 32 row loads in
+// flight, the same left-to-right comparisons per label, and one 16-byte
+// store of the 8 labels' bf16 values (the per-label kernel's 2-byte stores
+// strided by L each touched a line of their own).
+template <typename T>
+__global__ void k_minpool8(const T *__restrict__ d, int W, int H, int L, float scale, int w4, int h4,
+                           uint16_t *__restrict__ out) {
+    typedef T t4 __attribute__((ext_vector_type(4)));
+    const int64_t n = (int64_t)w4 * h4 * (L / 8);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int X = (int)(i % w4);
+        const int64_t r = i / w4;
+        const int Y = (int)(r % h4);
+        const int lg = (int)(r / h4);
+        const int y1 = min(Y * 4 + 4, H);
+        uint32_t packed[4];
+#pragma unroll
+        for (int k = 0; k < 8; k += 2) {
+            float mn[2] = {3.4e38f, 3.4e38f};
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const T *plane = d + (int64_t)(8 * lg + k + j) * H * W + X * 4;
+                for (int yy = Y * 4; yy < y1; ++yy) {
+                    const t4 v4 = *reinterpret_cast<const t4 *>(plane + (int64_t)yy * W);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float v = (float)v4[e];
+                        mn[j] = v < mn[j] ? v : mn[j];
+                    }
+                }
+            }
+            packed[k / 2] = (uint32_t)f2bf(mn[0] * scale) | ((uint32_t)f2bf(mn[1] * scale) << 16);
+        }
+        *reinterpret_cast<uint4 *>(out + ((int64_t)Y * w4 + X) * L + 8 * lg) =
+            uint4{packed[0], packed[1], packed[2], packed[3]};
+    }
+}
+
 // Training target: level-0 outboxes (oracle layout [l][d][y][x]) averaged over
 // f x f blocks, scaled, NHWC f32 [h][w][4L] with channel d*L + l.
 __global__ void k_block_mean_msgs(const float *__restrict__ m, int W, int H, int L, int f, float scale, int w4,
@@ -653,11 +692,29 @@ int ginf_conv3x3_fwd(const uint16_t *x, const uint16_t *w, const float *bias, in
         case 2: GINF_CONV_PF(2, 3); break;
         case 3: GINF_CONV_PF(3, 3); break;
         case 4: GINF_CONV_PF(4, 3); break;
+        case 5: GINF_CONV_PF(5, 1); break;
+        case 6: GINF_CONV_PF(6, 1); break;
+        case 7: GINF_CONV_PF(7, 1); break;
+        case 8: GINF_CONV_PF(8, 1); break;
         default:
             hipLaunchKernelGGL(k_conv3x3, grid, block, 0, st, x, w, bias, N, H, W, Cin, Cout, relu, y_bf16, y_f32);
     }
 #undef GINF_CONV_PF
     return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+}
+
+int ginf_convnet_fwd(int layers, const int *chans, const uint16_t *const *w, const float *const *b, const uint16_t *x,
+                     int N, int H, int W, uint16_t *act0, uint16_t *act1, float *out, void *stream) {
+    if (layers < 1 || !chans || !w || !b || !x || !out || (layers > 1 && (!act0 || !act1))) return GINF_ERR_INVALID;
+    for (int k = 0; k < layers; ++k) {
+        const uint16_t *in = k == 0 ? x : (k % 2 ? act0 : act1);
+        const bool last = k == layers - 1;
+        uint16_t *yb = last ? nullptr : (k % 2 ? act1 : act0);
+        const int rc = ginf_conv3x3_fwd(in, w[k], b[k], N, H, W, chans[k], chans[k + 1], last ? 0 : 1, yb,
+                                        last ? out : nullptr, stream);
+        if (rc != GINF_OK) return rc;
+    }
+    return GINF_OK;
 }
 
 int ginf_conv3x3_bwd_weight(const uint16_t *x, const uint16_t *dz, int N, int H, int W, int Cin, int Cout, float *dw,
@@ -764,6 +821,16 @@ int ginf_minpool_costs(const void *costs, int dtype, int W, int H, int L, int f,
     const int64_t n = (int64_t)w4 * h4 * L;
     // 16-byte row segments: every row and plane starts 16-byte aligned
     const bool vec4 = f == 4 && W % 4 == 0 && ((uintptr_t)costs & 15) == 0;
+    if (vec4 && L % 8 == 0 && ((uintptr_t)out & 15) == 0) {
+        const int64_t n8 = (int64_t)w4 * h4 * (L / 8);
+        if (dtype == GINF_F32)
+            hipLaunchKernelGGL(k_minpool8<float>, dim3(nblk(n8)), dim3(256), 0, (hipStream_t)stream,
+                               (const float *)costs, W, H, L, scale, w4, h4, out);
+        else
+            hipLaunchKernelGGL(k_minpool8<int>, dim3(nblk(n8)), dim3(256), 0, (hipStream_t)stream, (const int *)costs,
+                               W, H, L, scale, w4, h4, out);
+        return hipGetLastError() == hipSuccess ? GINF_OK : GINF_ERR_HIP;
+    }
     if (dtype == GINF_F32)
         hipLaunchKernelGGL(k_minpool<float>, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, (const float *)costs, W,
                            H, L, f, scale, w4, h4, out, vec4);

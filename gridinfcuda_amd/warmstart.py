@@ -116,6 +116,24 @@ class ConvNet:
         self.shape = (N, H, W)
         return self.out
 
+    def infer(self, x_bf16, N: int, H: int, W: int, out):
+        """Inference forward in one library call (ginf_convnet_fwd): the
+        same kernels and results as forward(), without keeping activations."""
+        torch = self.torch
+        hid = max(self.chans[1:-1]) if self.layers > 1 else 1
+        a0 = self._buf(("inf", 0), (N, H, W, hid), torch.int16)
+        a1 = self._buf(("inf", 1), (N, H, W, hid), torch.int16)
+        key = tuple(t.data_ptr() for t in self.wb + self.b)
+        if getattr(self, "_ptrs_key", None) != key:
+            n = self.layers
+            self._c_chans = (ctypes.c_int * (n + 1))(*self.chans)
+            self._c_w = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.wb])
+            self._c_b = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.b])
+            self._ptrs_key = key
+        _ck(G.lib().ginf_convnet_fwd(self.layers, self._c_chans, self._c_w, self._c_b, _vp(x_bf16), N, H, W, _vp(a0),
+                                    _vp(a1), _vp(out), self.stream()))
+        return out
+
     def backward(self, dout):
         """dout: FP32 gradient of the loss w.r.t. the output.  Fills self.gw/self.gb."""
         torch = self.torch
@@ -273,8 +291,11 @@ class Pipeline:
     def plain(self):
         self.plan.run(self.costs.data_ptr(), self.labels.data_ptr(), self.stream())
 
-    def target(self, out_slot):
-        self.plan.block_mean_msgs(0, POOL, self.wl.msg_scale, out_slot.data_ptr(), self.stream())
+    def target(self, out_slot, level=0):
+        """Outboxes of `level` after the plain run, averaged over the blocks
+        of level nodes that one 1/4-resolution cell covers (4x4 at level 0,
+        2x2 at level 1, one node at level 2)."""
+        self.plan.block_mean_msgs(level, POOL >> level, self.wl.msg_scale, out_slot.data_ptr(), self.stream())
 
     def energy(self) -> float:
         wl = self.wl
@@ -285,11 +306,16 @@ class Pipeline:
 # ---------------------------------------------------------------- training
 
 def train(wl: Workload, n_instances=20000, batch=16, lr=2e-3, reuse=2, seed_base=1_000_000, ckpt=None,
-          log=print, resume=None, time_budget_s=None):
+          log=print, resume=None, time_budget_s=None, target_level=0):
     """Online training: each batch of fresh seeded instances is labelled by
-    plain HBP on the GPU and used for `reuse` Adam steps."""
+    plain HBP on the GPU and used for `reuse` Adam steps.  The targets are
+    plain HBP's outboxes of `target_level` (0: the converged finest level,
+    averaged over 4x4 blocks; 2: the 1/4-resolution level at the end of its
+    half-sweeps, the state that plain HBP hands to level 1)."""
     import torch
     net = ConvNet.load(resume) if resume else ConvNet(wl.L)
+    if resume:
+        target_level = net.meta.get("target_level", 0)
     pipe = Pipeline(wl)
     xin = torch.empty((batch, wl.h4, wl.w4, wl.L), dtype=torch.int16, device="cuda")
     tgt = torch.empty((batch, wl.h4, wl.w4, 4 * wl.L), dtype=torch.float32, device="cuda")
@@ -311,7 +337,7 @@ def train(wl: Workload, n_instances=20000, batch=16, lr=2e-3, reuse=2, seed_base
             pipe.load_costs(lft, rgt)
             pipe.net_input(xin[k])
             pipe.plain()
-            pipe.target(tgt[k])
+            pipe.target(tgt[k], target_level)
         nbatches = max(1, len(seeds) // batch)
         warm = min(20, nbatches // 5 + 1)
         step_lr = lr * min(1.0, (pos / batch) / warm) * (0.1 ** (pos / max(1, len(seeds))))
@@ -326,12 +352,13 @@ def train(wl: Workload, n_instances=20000, batch=16, lr=2e-3, reuse=2, seed_base
         if len(history) % 25 == 1 or pos >= len(seeds):
             log(f"  instances {done:6d}  loss {lv:.5f}  lr {step_lr:.2e}  {time.time() - t0:.1f}s")
         if ckpt and len(history) % 200 == 0:
-            net.save(ckpt, {"instances": done, "workload": wl.key()})
+            net.save(ckpt, {"instances": done, "workload": wl.key(), "target_level": target_level})
         if time_budget_s and time.time() - t0 > time_budget_s:
             log(f"  time budget reached after {done} instances")
             break
     if ckpt:
-        net.save(ckpt, {"instances": done, "workload": wl.key(), "loss": history[-1][1]})
+        net.save(ckpt, {"instances": done, "workload": wl.key(), "loss": history[-1][1],
+                        "target_level": target_level})
     return net, history
 
 
@@ -366,7 +393,7 @@ class WarmStart:
         if self.net is not None:
             _ck(G.lib().ginf_minpool_costs(_vp(costs), G.F32, wl.W, wl.H, wl.L, POOL, IN_SCALE, _vp(self.xin),
                                           ctypes.c_void_p(self.stream())))
-            self.net.forward(self.xin, 1, wl.h4, wl.w4, out=self.pred)
+            self.net.infer(self.xin, 1, wl.h4, wl.w4, self.pred)
         else:
             self.pred.zero_()
         shift = 2 - seed_level
@@ -397,10 +424,10 @@ def _timed(torch, fn, reps=5):
 # seed level -1 is plain HBP from zero messages on all 5 levels (a baseline
 # with fewer iterations than the reference's 10 per level).
 SCHEDULES = [(0, (4,)), (0, (6,)), (0, (8,)), (0, (10,)), (0, (12,)), (0, (14,)),
-             (1, (2, 4)), (1, (4, 4)), (1, (4, 6)), (1, (6, 6)), (1, (6, 8)), (1, (0, 10)), (1, (4, 10)),
-             (1, (6, 10)), (1, (10, 10)),
-             (2, (4, 4, 6)), (2, (6, 6, 6)), (2, (6, 6, 8)), (2, (0, 10, 10)), (2, (2, 10, 10)), (2, (4, 10, 10)),
-             (2, (6, 10, 10)), (2, (10, 10, 10)),
+             (1, (2, 4)), (1, (4, 4)), (1, (4, 6)), (1, (6, 6)), (1, (6, 8)), (1, (0, 10)), (1, (2, 10)), (1, (3, 10)),
+             (1, (4, 10)), (1, (6, 10)), (1, (8, 10)), (1, (10, 10)), (1, (4, 8)), (1, (6, 9)),
+             (2, (4, 4, 6)), (2, (6, 6, 6)), (2, (6, 6, 8)), (2, (0, 8, 10)), (2, (0, 10, 8)), (2, (0, 10, 10)),
+             (2, (1, 10, 10)), (2, (2, 8, 10)), (2, (2, 10, 10)), (2, (4, 10, 10)), (2, (6, 10, 10)), (2, (10, 10, 10)),
              (-1, (4, 4, 4, 4, 4)), (-1, (6, 6, 6, 6, 6)), (-1, (10, 10, 10, 6, 6)), (-1, (10, 10, 10, 10, 8))]
 
 
@@ -466,13 +493,16 @@ def main(argv=None):
     ap.add_argument("--model", default=None)
     ap.add_argument("--resume", default=None)
     ap.add_argument("--budget", type=float, default=None, help="training time budget in seconds")
+    ap.add_argument("--target-level", type=int, default=0, choices=[0, 1, 2],
+                    help="train: pyramid level whose plain-HBP outboxes are the targets")
     ap.add_argument("--eval-seeds", type=int, default=8)
     ap.add_argument("--out", default=None, help="eval: write JSON records here")
     a = ap.parse_args(argv)
     wl = Workload(a.W, a.H, a.L, G.SM_POTTS if a.potts else G.SM_TRUNC_LINEAR, a.lam, a.tau)
     path = a.model or default_model_path(wl)
     if a.cmd == "train":
-        net, hist = train(wl, a.instances, a.batch, a.lr, a.reuse, ckpt=path, resume=a.resume, time_budget_s=a.budget)
+        net, hist = train(wl, a.instances, a.batch, a.lr, a.reuse, ckpt=path, resume=a.resume, time_budget_s=a.budget,
+                          target_level=a.target_level)
         print(json.dumps({"model": path, "params": net.params(), "instances": hist[-1][0], "final_loss": hist[-1][1],
                           "first_loss": hist[0][1]}))
     else:

@@ -126,7 +126,7 @@ class BatchedWarmStart:
                 for j, k in enumerate(idx):
                     _ck(G.lib().ginf_minpool_costs(_vp(self.costs[k]), G.F32, wl.W, wl.H, wl.L, POOL, IN_SCALE,
                                                   _vp(self.xin[c][j]), ctypes.c_void_p(st)))
-                self.nets[c].forward(self.xin[c], len(idx), wl.h4, wl.w4, out=self.pred[c])
+                self.nets[c].infer(self.xin[c], len(idx), wl.h4, wl.w4, self.pred[c])
             else:
                 self.pred[c].zero_()
             for j, k in enumerate(idx):

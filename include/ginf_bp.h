@@ -199,6 +199,14 @@ int ginf_bp_plan_seed(ginf_bp_plan plan, int level, const float *pred_dev, int p
  * [Cout][3][3][Cin], bf16 passed as uint16_t.  Cin % 32 == 0, Cout % 32 == 0. */
 int ginf_conv3x3_fwd(const uint16_t *x, const uint16_t *w, const float *bias, int N, int H, int W, int Cin, int Cout,
                      int relu, uint16_t *y_bf16, float *y_f32, void *stream);
+/* A whole network forward in one call: layer k is ginf_conv3x3_fwd from
+ * chans[k] to chans[k+1] channels with weights w[k] and bias b[k] (host
+ * arrays of device pointers), ReLU and a bf16 store between layers, FP32
+ * output without ReLU after the last.  Layer inputs alternate between act0
+ * and act1 (each N*H*W*max hidden channels bf16; unused for one layer).  For
+ * inference loops, where one host call per layer costs more than the layer. */
+int ginf_convnet_fwd(int layers, const int *chans, const uint16_t *const *w, const float *const *b, const uint16_t *x,
+                     int N, int H, int W, uint16_t *act0, uint16_t *act1, float *out, void *stream);
 /* dW [Cout][3][3][Cin] and db [Cout] from the layer input x and the
  * pre-activation gradient dz (both bf16 NHWC); deterministic. */
 int ginf_conv3x3_bwd_weight(const uint16_t *x, const uint16_t *dz, int N, int H, int W, int Cin, int Cout, float *dw,

@@ -28,7 +28,8 @@ def rel(a, b):
 
 @pytest.mark.parametrize("N,H,W,Ci,Co", [(2, 7, 9, 32, 32), (1, 5, 33, 64, 32), (3, 4, 6, 32, 256), (1, 6, 7, 96, 384),
                                          (2, 3, 17, 32, 64), (1, 9, 5, 256, 32), (4, 21, 45, 32, 256),
-                                         (2, 37, 70, 64, 32)])
+                                         (2, 37, 70, 64, 32), (1, 5, 11, 160, 32), (1, 4, 9, 192, 64),
+                                         (1, 6, 6, 224, 32), (1, 3, 7, 288, 32)])
 def test_conv_forward_and_gradients(N, H, W, Ci, Co):
     torch = torch_cuda()
     say(f"conv N={N} H={H} W={W} Ci={Ci} Co={Co}")
@@ -91,6 +92,23 @@ def test_conv_weight_gradient_shapes(N, H, W, Ci, Co):
     assert rel(db.cpu().numpy(), rdb) < 1e-5
 
 
+def test_infer_matches_forward():
+    """The one-call network forward (ginf_convnet_fwd) gives the per-layer
+    forward's output bit for bit."""
+    torch = torch_cuda()
+    from gridinfcuda_amd.warmstart import ConvNet
+    say("one-call network forward")
+    net = ConvNet(64, seed=5)
+    rng = np.random.default_rng(6)
+    N, H, W = 2, 13, 21
+    x = dev_bf16(torch, rng.standard_normal((N, H, W, 64)))
+    a = net.forward(x, N, H, W).clone()
+    b = torch.empty_like(a)
+    net.infer(x, N, H, W, b)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
 def test_network_gradients_match_oracle():
     """Six-layer net (L=32 -> 32 x5 -> 128) forward + backward on the GPU vs the
     oracle with the same bf16 rounding points."""
@@ -140,6 +158,31 @@ def test_adam_matches_numpy():
         m2 = 0.999 * m2 + 0.001 * g * g
         w = w - 1e-2 * (m1 / (1 - 0.9 ** step)) / (np.sqrt(m2 / (1 - 0.999 ** step)) + 1e-8)
     np.testing.assert_allclose(wd.cpu().numpy(), w, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("W,H,L,dt", [(64, 45, 32, "f32"), (128, 37, 64, "f32"), (64, 44, 20, "f32"),
+                                      (70, 45, 32, "f32"), (96, 42, 24, "i32"), (68, 40, 16, "i32")])
+def test_minpool_shapes(W, H, L, dt):
+    """ginf_minpool_costs against a numpy 4x4 min-pool for every kernel form:
+    8 labels per thread (W % 4 == 0, L % 8 == 0), 16-byte rows per label,
+    and the per-element fallback; partial blocks at the bottom edge."""
+    torch = torch_cuda()
+    from gridinfcuda_amd.warmstart import POOL, IN_SCALE
+    say(f"minpool W={W} H={H} L={L} {dt}")
+    rng = np.random.default_rng(W * H + L)
+    c = rng.integers(0, 41, size=(L, H, W))
+    costs = torch.from_numpy(c.astype(np.float32 if dt == "f32" else np.int32)).cuda()
+    w4, h4 = (W + 3) // 4, (H + 3) // 4
+    out = torch.empty((h4, w4, L), dtype=torch.int16, device="cuda")
+    G.check(G.lib().ginf_minpool_costs(vp(costs), G.F32 if dt == "f32" else G.I32, W, H, L, POOL,
+                                       ctypes.c_float(IN_SCALE), vp(out),
+                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    torch.cuda.synchronize()
+    pad = np.full((L, h4 * 4, w4 * 4), np.inf)
+    pad[:, :H, :W] = c
+    ref = pad.reshape(L, h4, 4, w4, 4).min(axis=(2, 4))
+    got = C.from_bf16_bits(out.cpu().numpy().view(np.uint16))
+    np.testing.assert_array_equal(got, C.bf16_round(np.transpose(ref, (1, 2, 0)).astype(np.float32) * np.float32(IN_SCALE)))
 
 
 def test_costs_pool_and_targets():
