@@ -33,6 +33,7 @@ EXPORTS = [
     "ginf_icm", "ginf_max_prior", "ginf_icm_host", "ginf_max_prior_host", "ginf_alpha_beta",
     "ginf_alpha_beta_host", "ginf_stream_probe", "ginf_bp_plan_set_compact_data", "ginf_bp_plan_compact_data",
     "ginf_batch_set_compact_data", "ginf_bp_plan_set_pairs", "ginf_bp_plan_pairs", "ginf_convnet_fwd",
+    "ginf_bp_plan_run_predicted", "ginf_bp_plan_prepare",
 ]
 
 
@@ -87,6 +88,8 @@ def lib() -> ctypes.CDLL:
         I64, F = ctypes.c_int64, ctypes.c_float
         L.ginf_bp_plan_run_schedule.argtypes = [VP, VP, VP, I, P(I), I, VP]
         L.ginf_bp_plan_seed.argtypes = [VP, I, VP, I, I, I, F, VP]
+        L.ginf_bp_plan_run_predicted.argtypes = [VP, VP, VP, I, P(I), VP, I, I, I, F, I, VP]
+        L.ginf_bp_plan_prepare.argtypes = [VP, VP, I, VP, F, VP]
         L.ginf_bp_plan_block_mean_msgs.argtypes = [VP, I, I, F, VP, VP]
         L.ginf_conv3x3_fwd.argtypes = [VP, VP, VP, I, I, I, I, I, I, VP, VP, VP]
         L.ginf_convnet_fwd.argtypes = [I, VP, VP, VP, VP, I, I, I, VP, VP, VP, VP]
@@ -183,6 +186,24 @@ class Plan:
     def seed(self, level: int, pred_ptr: int, pw: int, ph: int, shift: int, scale: float, stream: int = 0) -> None:
         check(lib().ginf_bp_plan_seed(self._h, level, ctypes.c_void_p(pred_ptr), pw, ph, shift, scale,
                                       ctypes.c_void_p(stream)))
+
+    def run_predicted(self, data_ptr: int, labels_ptr: int, start_level: int, iters, pred_ptr: int, pw: int,
+                      ph: int, shift: int, scale: float, stream: int = 0, prepared: bool = False) -> None:
+        """seed(start_level, pred, ...) + run_schedule(..., seeded=True) in one
+        call, the first half-sweep reading the prediction directly; with
+        prepared, the level data of the last prepare() is used."""
+        arr = (ctypes.c_int * self.desc.num_levels)(*([int(v) for v in iters] + [0] * (self.desc.num_levels - len(iters))))
+        check(lib().ginf_bp_plan_run_predicted(self._h, ctypes.c_void_p(data_ptr), ctypes.c_void_p(labels_ptr),
+                                               start_level, arr, ctypes.c_void_p(pred_ptr), pw, ph, shift, scale,
+                                               1 if prepared else 0, ctypes.c_void_p(stream)))
+
+    def prepare(self, data_ptr: int, start_level: int, minpool_ptr: int = 0, minpool_scale: float = 0.0,
+                stream: int = 0) -> None:
+        """Level data of levels 0..start_level (and, with minpool_ptr, the
+        min-pooled network input from the same pass)."""
+        check(lib().ginf_bp_plan_prepare(self._h, ctypes.c_void_p(data_ptr), start_level,
+                                         ctypes.c_void_p(minpool_ptr) if minpool_ptr else None, minpool_scale,
+                                         ctypes.c_void_p(stream)))
 
     def block_mean_msgs(self, level: int, f: int, scale: float, out_ptr: int, stream: int = 0) -> None:
         check(lib().ginf_bp_plan_block_mean_msgs(self._h, level, f, scale, ctypes.c_void_p(out_ptr),

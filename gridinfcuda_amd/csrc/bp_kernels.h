@@ -64,6 +64,10 @@ struct UpsampleArgs {
 // costs in 16-bit storage when it is 0 (quad kernel only).
 hipError_t launch_sweep(int dtype, const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
                         hipStream_t st, bool data16 = false);
+// The first half-sweep of a seeded level with its incoming messages read from
+// a prediction (SweepArgs::pred_*, in_base = the prediction; sweep_seeded.hip):
+// quad layout, FP32, L % 4 == 0.
+hipError_t launch_sweep_seeded(const SweepShape &sh, const SweepArgs &a, bool labels, bool d16, hipStream_t st);
 // Per-family launchers (sweep_chunked_*.hip, sweep_quad.hip, sweep_general.hip).
 hipError_t launch_sweep_chunked_f32(const SweepShape &sh, const SweepArgs &a, bool zeroin, bool labels,
                                     hipStream_t st);
@@ -98,6 +102,13 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
                                const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
                                void *d11, hipStream_t st, bool data16 = false, int *flag = nullptr,
                                const int *only_if = nullptr);
+// The label-quad relayout that also writes the costs min-pooled over 4 x 4
+// blocks, times `scale`, as BF16 NHWC [(H+3)/4][(W+3)/4][L] into mp (the
+// warm-start network's input); hipErrorNotSupported for other layouts.
+hipError_t launch_relayout_quad_mp(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
+                                   const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1,
+                                   void *d10, void *d11, hipStream_t st, bool data16, int *flag, uint16_t *mp,
+                                   float scale);
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
                           const LevelGeom &cg, int C, int G, hipStream_t st, bool data16 = false, int *flag = nullptr,
                           const int *only_if = nullptr);

@@ -6,6 +6,7 @@
 #include <type_traits>
 
 #include "bp_kernels.h"
+#include "conv_common.h"
 
 namespace ginf {
 namespace {
@@ -299,18 +300,11 @@ __global__ __launch_bounds__(256) void k_energy_final(const double *partials, in
 // one z-block per plane, and the conditional redo one z-block in all, so that
 // the redo's blocks are few when they have nothing to do.
 template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src, int W, int H, int L, int C,
-                                                       LevelGeom g0, S *d00, S *d01, int coarse, LevelGeom g1, S *d10,
-                                                       S *d11, int *flag, const int *only_if) {
-    if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // column pair
-    const int Y = blockIdx.y;
+__device__ __forceinline__ void relayout_quad_pair(const T *__restrict__ src, int W, int H, int L, int C,
+                                                   const LevelGeom &g0, S *d00, S *d01, int coarse, const LevelGeom &g1,
+                                                   S *d10, S *d11, int k, int Y, int i, bool &bad, T (&v)[4][2][2]) {
     const int x0 = 2 * k, y0 = 2 * Y;
-    if (x0 >= W) return;
     const bool ex = x0 + 1 < W, ey = y0 + 1 < H;
-    bool bad = false;
-    for (int i = blockIdx.z; i < C; i += gridDim.z) {
-    T v[4][2][2];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int l = 4 * i + q;
@@ -357,6 +351,76 @@ __global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src
         }
         *(s4 *)(dst + elem_index(slot, i, 0, C, 4)) = o;
     }
+}
+
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src, int W, int H, int L, int C,
+                                                       LevelGeom g0, S *d00, S *d01, int coarse, LevelGeom g1, S *d10,
+                                                       S *d11, int *flag, const int *only_if) {
+    if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // column pair
+    const int Y = blockIdx.y;
+    if (2 * k >= W) return;
+    bool bad = false;
+    for (int i = blockIdx.z; i < C; i += gridDim.z) {
+        T v[4][2][2];
+        relayout_quad_pair<T, S>(src, W, H, L, C, g0, d00, d01, coarse, g1, d10, d11, k, Y, i, bad, v);
+    }
+    report_bad(bad, flag);
+}
+
+// The same relayout over four rows per block (two row pairs), which also
+// writes the warm-start network's input: the costs min-pooled over 4 x 4
+// pixel blocks, times `scale`, rounded to BF16, NHWC [h4][w4][L] -- the values
+// of ginf_minpool_costs, without reading the cost volume a second time.  A
+// lane's two row pairs give the minimum over 2 columns x 4 rows; the
+// neighbouring lane (the next column pair) completes the 4 x 4 block.
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_relayout_quad_mp(const T *__restrict__ src, int W, int H, int L, int C,
+                                                          LevelGeom g0, S *d00, S *d01, int coarse, LevelGeom g1,
+                                                          S *d10, S *d11, int *flag, uint16_t *__restrict__ mp, int w4,
+                                                          float scale) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // column pair
+    const int Y4 = blockIdx.y;
+    const bool live = 2 * k < W;  // no early exit: lanes k and k ^ 1 exchange their minima
+    bool bad = false;
+    for (int i = blockIdx.z; i < C; i += gridDim.z) {
+        float mn[4] = {3.4e38f, 3.4e38f, 3.4e38f, 3.4e38f};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int Y = 2 * Y4 + h;
+            if (!live || 2 * Y >= H) continue;
+            T v[4][2][2];
+            relayout_quad_pair<T, S>(src, W, H, L, C, g0, d00, d01, coarse, g1, d10, d11, k, Y, i, bad, v);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                        if ((r == 0 || 2 * Y + 1 < H) && (c == 0 || 2 * k + 1 < W)) {
+                            const float f = (float)v[q][r][c];
+                            mn[q] = f < mn[q] ? f : mn[q];
+                        }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float o = __shfl_xor(mn[q], 1);
+            mn[q] = o < mn[q] ? o : mn[q];
+        }
+        if (live && (k & 1) == 0) {
+            uint16_t *dst = mp + ((int64_t)Y4 * w4 + (k >> 1)) * L + 4 * i;
+            if (4 * i + 3 < L && (L & 3) == 0) {
+                *reinterpret_cast<uint2 *>(dst) = uint2{(uint32_t)f2bf(mn[0] * scale) |
+                                                             ((uint32_t)f2bf(mn[1] * scale) << 16),
+                                                         (uint32_t)f2bf(mn[2] * scale) |
+                                                             ((uint32_t)f2bf(mn[3] * scale) << 16)};
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (4 * i + q < L) dst[q] = f2bf(mn[q] * scale);
+            }
+        }
     }
     report_bad(bad, flag);
 }
@@ -422,6 +486,23 @@ hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, cons
     else
         hipLaunchKernelGGL(k_relayout<int>, grid, block, 0, st, (const int *)src, W, H, L, ly, g, (int *)dst0,
                            (int *)dst1);
+    return hipGetLastError();
+}
+
+hipError_t launch_relayout_quad_mp(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
+                                   const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1,
+                                   void *d10, void *d11, hipStream_t st, bool data16, int *flag, uint16_t *mp,
+                                   float scale) {
+    if (!(ly.interleave && ly.G == 4)) return hipErrorNotSupported;
+    const dim3 grid((unsigned)((W + 511) / 512), (unsigned)((H + 3) / 4), (unsigned)ly.C), block(256);
+    if (ly.C > 65535 || (H + 3) / 4 > 65535) return hipErrorInvalidValue;
+    const int w4 = (W + 3) / 4;
+#define GINF_RQM(TT, SS)                                                                                         \
+    hipLaunchKernelGGL((k_relayout_quad_mp<TT, SS>), grid, block, 0, st, (const TT *)src, W, H, L, ly.C, g0,      \
+                       (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, mp, w4, scale)
+    if (data16) { if (dtype == 0) GINF_RQM(float, uint16_t); else GINF_RQM(int, uint16_t); }
+    else { if (dtype == 0) GINF_RQM(float, float); else GINF_RQM(int, int); }
+#undef GINF_RQM
     return hipGetLastError();
 }
 

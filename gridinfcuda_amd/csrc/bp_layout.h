@@ -89,6 +89,13 @@ struct SweepArgs {
     uint32_t data_plane_bytes;  // bytes between data-cost planes in 16-bit storage (half of plane_bytes;
                                 // 16-bit element offsets are the T offsets halved)
     const int *fit_flag;     // 16-bit data path: 0 = the level data is stored in 16 bits, else in T
+    // Prediction-sourced first half-sweep of a seeded level (quad kernel, FP32):
+    // the incoming message from neighbour n in direction od is
+    // pred[min(ny >> pred_shift, pred_h - 1)][min(nx >> pred_shift, pred_w - 1)][od * L + l]
+    // * pred_inv_scale for an interior n, 0 for a border n.  in_base then
+    // points at the prediction and in_span is its size in bytes.
+    int pred_w, pred_h, pred_shift;
+    float pred_inv_scale;
 };
 
 }  // namespace ginf

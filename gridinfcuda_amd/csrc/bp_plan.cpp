@@ -143,6 +143,7 @@ struct ginf_bp_plan_s {
     // in [0, 65535] (flag_dev stays 0), else it redoes the level data in T.
     bool compact = true;
     bool compact_last = false;
+    int prepared_start = -1;  // highest level whose data the last plan_prepare built
     int *flag_dev = nullptr;
     // fused pairs of half-sweeps (ginf_bp_plan_set_pairs) and, per level, the
     // second set of colour-0 outbox arrays they alternate with
@@ -337,29 +338,43 @@ int plan_alloc_pairs(ginf_bp_plan p) {
     return GINF_OK;
 }
 
-// The HBP driver.  iters[t] half-sweeps run at level t for t <= start;
-// levels above `start` are skipped.  When `seeded`, level `start` begins from
-// the outboxes already in its arrays (written by ginf_bp_plan_seed) instead
-// of zeros.  ginf_bp_plan_run is start = levels-1, iters[t] = the
-// descriptor's count, not seeded: decodeHbp (src/bp.cc:86-180).
-int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStream_t st, int start,
-                const int *iters, bool seeded) {
+// A prediction that the first half-sweep of the seeded level reads in place
+// of seeded outboxes (ginf_bp_plan_run_predicted).
+struct PredSource {
+    const float *pred;
+    int pw, ph, shift;
+    float inv_scale;
+};
+
+// Level data of levels 0..start from the caller's costs: level 0 (and level
+// 1) in one pass over the caller's array, then the coarser levels.  With
+// 16-bit storage every value is checked on the device; if any has no exact
+// 16-bit form, conditional launches redo the level data in T and the sweeps
+// (which read the same flag) take it from there.  Both forms hold the same
+// values, and no host synchronisation is needed.  With mp (label-quad layout
+// only) the first pass also writes the costs min-pooled over 4 x 4 blocks,
+// times mp_scale, as BF16 NHWC (the warm-start network's input); *mp_done
+// tells whether it did.
+int plan_prepare(ginf_bp_plan p, const void *data_dev, hipStream_t st, int start, uint16_t *mp = nullptr,
+                 float mp_scale = 0.f, bool *mp_done = nullptr) {
     using namespace ginf;
     const ginf_bp_desc &d = p->desc;
     const StateLayout ly = state_layout(p->shape, d.num_labels);
-    const int dt = d.dtype, C = ly.C, G = ly.G, T = p->levels;
+    const int dt = d.dtype, C = ly.C, G = ly.G;
     const int W = d.width, H = d.height, L = d.num_labels;
-    // Level 0 (and level 1) costs in one pass over the caller's array, then
-    // the coarser levels.  With 16-bit storage every value is checked on the
-    // device; if any has no exact 16-bit form, conditional launches redo the
-    // level data in T and the sweeps (which read the same flag) take it from
-    // there.  Both forms hold the same values, and no host synchronisation is
-    // needed.
-    const bool two = T > 1;
+    const bool two = start >= 1;  // a run that starts at level 0 needs no level-1 sums
+    const bool with_mp = mp && ly.interleave && ly.G == 4;
+    if (mp_done) *mp_done = with_mp;
     auto prepare = [&](bool d16, int *flag, const int *only_if) -> int {
-        GINF_HIP_TRY(launch_relayout_pyr(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0], p->lv[0].data[1],
-                                         two, p->geom[two ? 1 : 0], p->lv[two ? 1 : 0].data[0],
-                                         p->lv[two ? 1 : 0].data[1], st, d16, flag, only_if));
+        if (with_mp && !only_if)
+            GINF_HIP_TRY(launch_relayout_quad_mp(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0],
+                                                 p->lv[0].data[1], two, p->geom[two ? 1 : 0],
+                                                 p->lv[two ? 1 : 0].data[0], p->lv[two ? 1 : 0].data[1], st, d16,
+                                                 flag, mp, mp_scale));
+        else
+            GINF_HIP_TRY(launch_relayout_pyr(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0],
+                                             p->lv[0].data[1], two, p->geom[two ? 1 : 0], p->lv[two ? 1 : 0].data[0],
+                                             p->lv[two ? 1 : 0].data[1], st, d16, flag, only_if));
         for (int t = 2; t <= start; ++t)
             GINF_HIP_TRY(launch_pyramid(dt, p->lv[t - 1].data[0], p->lv[t - 1].data[1], p->geom[t - 1],
                                         p->lv[t].data[0], p->lv[t].data[1], p->geom[t], C, G, st, d16, flag,
@@ -375,6 +390,27 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
         return rc;
     }
     p->compact_last = d16;
+    p->prepared_start = start;
+    return GINF_OK;
+}
+
+// The HBP driver.  iters[t] half-sweeps run at level t for t <= start;
+// levels above `start` are skipped.  When `seeded`, level `start` begins from
+// the outboxes already in its arrays (written by ginf_bp_plan_seed) instead
+// of zeros.  ginf_bp_plan_run is start = levels-1, iters[t] = the
+// descriptor's count, not seeded: decodeHbp (src/bp.cc:86-180).  With
+// `prepared` the level data of the last plan_prepare is used as it is.
+int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipStream_t st, int start,
+                const int *iters, bool seeded, const PredSource *pred = nullptr, bool prepared = false) {
+    using namespace ginf;
+    const ginf_bp_desc &d = p->desc;
+    const StateLayout ly = state_layout(p->shape, d.num_labels);
+    const int dt = d.dtype, C = ly.C, G = ly.G;
+    const int W = d.width, H = d.height, L = d.num_labels;
+    if (!prepared) {
+        if (int rc = plan_prepare(p, data_dev, st, start)) return rc;
+    }
+    const bool d16 = p->compact_last;
     for (int t = start; t >= 0; --t) {
         const LevelGeom &g = p->geom[t];
         const int64_t per = (int64_t)C * g.nplane * G;
@@ -464,8 +500,20 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
         // half-sweeps: 0.044 ms unfused, 0.051 ms fused, 48 tiles).
         const bool pairs = pairs_on(p) && p->alt_pool && g.w >= 3 && g.h >= 3 &&
                            ginf::pair_tiles_f32(p->shape.C, g.whp, g.h) >= kPairMinTiles;
+        // With a prediction source (top level, N >= 2) half-sweep 0 reads the
+        // prediction and half-sweeps 0 and 1 run unfused; the pairs after them
+        // read the level's own colour-B arrays first, so their count must be
+        // even as for a seeded top level.
+        const bool from_pred = top && pred != nullptr;
         int npairs = pairs ? N / 2 : 0, first_pair = 0;
-        if (npairs % 2 == 1 && top && seeded) { first_pair = 2; --npairs; }
+        if (from_pred && npairs > 0) {
+            first_pair = 2;
+            --npairs;
+            if (npairs % 2 == 1) { first_pair = 4; --npairs; }
+        } else if (npairs % 2 == 1 && top && seeded) {
+            first_pair = 2;
+            --npairs;
+        }
         int k = 0;
         bool prev_alt = false;
         for (int i = 0; i < N;) {
@@ -477,14 +525,25 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
             ginf_bp_plan_s::Timed tm{nullptr, nullptr, 0.0};
             // Only the steady-state kernel instance (no zero-input, no labels) is
             // timed, so the figure matches a per-kernel profiler summary.
-            if (p->timing && !zeroin && !labels_i && !labels_j) {
+            if (p->timing && !zeroin && !labels_i && !labels_j && !(from_pred && i == 0)) {
                 tm.beg = take_event(p);
                 tm.end = take_event(p);
                 tm.updates = (double)interior_of_color(g.w, g.h, a.color) * L;
                 if (pair) tm.updates += (double)interior_of_color(g.w, g.h, 1 - a.color) * L;
                 if (tm.beg) GINF_HIP_TRY(hipEventRecord(tm.beg, st));
             }
-            if (pair) {
+            if (from_pred && i == 0) {
+                SweepArgs ap = a;
+                ap.src = kSrcSame;
+                ap.in_base = pred->pred;
+                ap.in_span = (uint32_t)((size_t)pred->pw * pred->ph * 4 * L * sizeof(float));
+                ap.in_plane_bytes = 16;  // label quad j of a prediction row
+                ap.pred_w = pred->pw;
+                ap.pred_h = pred->ph;
+                ap.pred_shift = pred->shift;
+                ap.pred_inv_scale = pred->inv_scale;
+                GINF_HIP_TRY(launch_sweep_seeded(p->shape, ap, labels_i, d16, st));
+            } else if (pair) {
                 SweepArgs aa = a;
                 SweepArgs b = make_args(i + 1);
                 if (k > 0 && prev_alt) aa.in_base = p->alt[t];  // the previous pair's colour-B outboxes
@@ -754,6 +813,50 @@ int ginf_bp_plan_seed(ginf_bp_plan p, int level, const float *pred_dev, int pw, 
     GINF_HIP_TRY(ginf::launch_inject(pred_dev, m, p->geom[level], ly.C, ly.G, ly.interleave, p->desc.num_labels, shift,
                                      pw, ph, 1.f / scale, (hipStream_t)stream));
     return GINF_OK;
+}
+
+int ginf_bp_plan_prepare(ginf_bp_plan p, const void *data_dev, int start_level, uint16_t *minpool_out,
+                         float minpool_scale, void *stream) {
+    if (!p || !data_dev) return set_error(GINF_ERR_INVALID, "NULL argument");
+    if (start_level < 0 || start_level >= p->levels) return set_error(GINF_ERR_INVALID, "start level %d", start_level);
+    bool done = false;
+    if (int rc = plan_prepare(p, data_dev, (hipStream_t)stream, start_level, minpool_out, minpool_scale, &done))
+        return rc;
+    if (minpool_out && !done) {
+        // layouts other than label quads: the separate min-pool kernel
+        const ginf_bp_desc &d = p->desc;
+        if (ginf_minpool_costs(data_dev, d.dtype, d.width, d.height, d.num_labels, 4, minpool_scale, minpool_out,
+                               stream) != GINF_OK)
+            return set_error(GINF_ERR_HIP, "min-pool kernel failed");
+    }
+    return GINF_OK;
+}
+
+int ginf_bp_plan_run_predicted(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, int start_level,
+                               const int *iters, const float *pred_dev, int pw, int ph, int shift, float scale,
+                               int flags, void *stream) {
+    if (!p || !data_dev || !labels_dev || !iters || !pred_dev) return set_error(GINF_ERR_INVALID, "NULL argument");
+    if (start_level < 0 || start_level >= p->levels) return set_error(GINF_ERR_INVALID, "start level %d", start_level);
+    for (int t = 0; t <= start_level; ++t)
+        if (iters[t] < 0) return set_error(GINF_ERR_INVALID, "iters[%d] < 0", t);
+    if (pw < 1 || ph < 1 || shift < 0 || shift > 8 || scale == 0.f)
+        return set_error(GINF_ERR_INVALID, "bad prediction arguments");
+    if (flags & ~GINF_RUN_PREPARED) return set_error(GINF_ERR_INVALID, "unknown flags 0x%x", flags);
+    const bool prepared = (flags & GINF_RUN_PREPARED) != 0;
+    if (prepared && p->prepared_start < start_level)
+        return set_error(GINF_ERR_INVALID, "level %d was not prepared (ginf_bp_plan_prepare)", start_level);
+    if (p->desc.dtype != GINF_F32) return set_error(GINF_ERR_UNSUPPORTED, "seeding needs FP32 messages");
+    const int L = p->desc.num_labels;
+    const bool direct = p->shape.quad && L % 4 == 0 && iters[start_level] >= 2 && ((uintptr_t)pred_dev & 15) == 0 &&
+                        (size_t)pw * ph * 4 * L * sizeof(float) < ((size_t)1 << 32);
+    if (!direct) {
+        // Other layouts, or fewer than two half-sweeps at the seeded level
+        // (where colour B's seeded outboxes stay in use): seed, then run.
+        if (int rc = ginf_bp_plan_seed(p, start_level, pred_dev, pw, ph, shift, scale, stream)) return rc;
+        return plan_run_ex(p, data_dev, labels_dev, (hipStream_t)stream, start_level, iters, true, nullptr, prepared);
+    }
+    const PredSource src{pred_dev, pw, ph, shift, 1.f / scale};
+    return plan_run_ex(p, data_dev, labels_dev, (hipStream_t)stream, start_level, iters, true, &src, prepared);
 }
 
 int ginf_bp_plan_block_mean_msgs(ginf_bp_plan p, int level, int f, float scale, float *out_dev, void *stream) {

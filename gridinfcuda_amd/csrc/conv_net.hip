@@ -16,25 +16,25 @@
 
 #include "ginf_bp.h"
 #include "bp_layout.h"
+#include "conv_common.h"
 
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f4 __attribute__((ext_vector_type(4)));
+using ginf::bf16x8;
+using ginf::f4;
+using ginf::bf2f;
+using ginf::f2bf;
+using ginf::load8;
 
-__device__ __forceinline__ float bf2f(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+}  // namespace
 
-// Round-to-nearest-even float -> bf16 (NaN kept quiet).
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+namespace ginf {
+// convnet_fused.hip
+hipError_t launch_convnet_fused(int layers, const int *chans, const uint16_t *const *w, const float *const *b,
+                                const uint16_t *x, int N, int H, int W, float *out, hipStream_t st);
+}  // namespace ginf
 
-__device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
-    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
-}
+namespace {
 
 // y = conv3x3(x, w) + bias (zero padding 1), optional ReLU.  Each wave owns
 // 16 consecutive pixels (flattened n,h,w) and 32 output channels (two MFMA
@@ -521,23 +521,32 @@ __global__ void k_minpool8(const T *__restrict__ d, int W, int H, int L, float s
         const int64_t r = i / w4;
         const int Y = (int)(r % h4);
         const int lg = (int)(r / h4);
-        const int y1 = min(Y * 4 + 4, H);
+        // All 32 row reads (8 labels x 4 rows) are issued before the first
+        // minimum: the rows are unrolled with a guard at the bottom edge
+        // instead of a loop bound that would serialise them.
+        t4 v4[8][4];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const T *plane = d + (int64_t)(8 * lg + j) * H * W + X * 4;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (Y * 4 + r < H) v4[j][r] = *reinterpret_cast<const t4 *>(plane + (int64_t)(Y * 4 + r) * W);
+        }
         uint32_t packed[4];
 #pragma unroll
         for (int k = 0; k < 8; k += 2) {
             float mn[2] = {3.4e38f, 3.4e38f};
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const T *plane = d + (int64_t)(8 * lg + k + j) * H * W + X * 4;
-                for (int yy = Y * 4; yy < y1; ++yy) {
-                    const t4 v4 = *reinterpret_cast<const t4 *>(plane + (int64_t)yy * W);
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (Y * 4 + r >= H) continue;
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        const float v = (float)v4[e];
+                        const float v = (float)v4[k + j][r][e];
                         mn[j] = v < mn[j] ? v : mn[j];
                     }
                 }
-            }
             packed[k / 2] = (uint32_t)f2bf(mn[0] * scale) | ((uint32_t)f2bf(mn[1] * scale) << 16);
         }
         *reinterpret_cast<uint4 *>(out + ((int64_t)Y * w4 + X) * L + 8 * lg) =
@@ -705,7 +714,12 @@ int ginf_conv3x3_fwd(const uint16_t *x, const uint16_t *w, const float *bias, in
 
 int ginf_convnet_fwd(int layers, const int *chans, const uint16_t *const *w, const float *const *b, const uint16_t *x,
                      int N, int H, int W, uint16_t *act0, uint16_t *act1, float *out, void *stream) {
-    if (layers < 1 || !chans || !w || !b || !x || !out || (layers > 1 && (!act0 || !act1))) return GINF_ERR_INVALID;
+    if (layers < 1 || !chans || !w || !b || !x || !out) return GINF_ERR_INVALID;
+    // The shapes the one-launch kernel is built for run there (same results).
+    const hipError_t fe = ginf::launch_convnet_fused(layers, chans, w, b, x, N, H, W, out, (hipStream_t)stream);
+    if (fe == hipSuccess) return GINF_OK;
+    if (fe != hipErrorNotSupported) return GINF_ERR_HIP;
+    if (layers > 1 && (!act0 || !act1)) return GINF_ERR_INVALID;
     for (int k = 0; k < layers; ++k) {
         const uint16_t *in = k == 0 ? x : (k % 2 ? act0 : act1);
         const bool last = k == layers - 1;

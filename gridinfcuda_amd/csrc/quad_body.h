@@ -52,7 +52,14 @@ __device__ __forceinline__ int quad_bcast(int v) {
 // soffset, the compiler reused SGPRs that a v_cmp in the labelling code had
 // just written, and the memory instruction then intermittently read the wrong
 // offset (nondeterministic message corruption, measured on gfx950).
-template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool use16>
+//   PRED  the incoming messages come from a network prediction at
+//         1/2^pred_shift of the level's resolution instead of the partner
+//         colour's outboxes (first half-sweep of a seeded level): each value is
+//         the prediction times pred_inv_scale, 0 from a border neighbour -- the
+//         outboxes ginf_bp_plan_seed would have stored, read where they come
+//         from (FP32 only; a lane's label quad is 16 contiguous bytes of the
+//         prediction row, L % 4 == 0).
+template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool use16, bool PRED = false>
 __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slot, const int d) {
     typedef T v4 __attribute__((ext_vector_type(4)));
     constexpr int LQ = LMAX / 4;
@@ -71,7 +78,14 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
     const uint32_t data_off = use16 ? self_off / 2u : self_off;  // 16-bit storage: every offset halves
     const uint32_t data_plane = use16 ? a.data_plane_bytes : a.plane_bytes;
     uint32_t in_off = 0;
-    if (!ZEROIN) {
+    bool in_live = true;
+    if constexpr (PRED) {
+        const int nx = x + kDX[d], ny = y + kDY[d];
+        const int od = (d + 2) & 3;
+        in_live = nx >= 1 && ny >= 1 && nx <= a.w - 2 && ny <= a.h - 2;
+        const int X = min(nx >> a.pred_shift, a.pred_w - 1), Y = min(ny >> a.pred_shift, a.pred_h - 1);
+        in_off = ((uint32_t)((Y * a.pred_w + X) * 4 + od) * (uint32_t)L) * 4u;
+    } else if (!ZEROIN) {
         int nx = x + kDX[d], ny = y + kDY[d];
         int csel;
         if (a.src == kSrcCoarse) { nx >>= 1; ny >>= 1; csel = (nx + ny) & 1; }
@@ -96,6 +110,10 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
             v4 m4 = v4(T(0));
             if (!ZEROIN)
                 m4 = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(irs, in_off + (uint32_t)j * a.in_plane_bytes, 0, GINF_QUAD_LOAD_POLICY));
+            if constexpr (PRED) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) m4[r] = in_live ? m4[r] * a.pred_inv_scale : T(0);
+            }
             v4 d4;
             if constexpr (use16) {
                 const auto h = __builtin_amdgcn_raw_buffer_load_b64(drs, data_off + (uint32_t)j * data_plane, 0, GINF_QUAD_LOAD_POLICY);
@@ -177,15 +195,15 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
 // The two storage forms are separate straight-line bodies; the uniform flag
 // picks one per wave (a branch inside the unrolled label loop would stop the
 // compiler from issuing the loads ahead).
-template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool D16 = false>
+template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool D16 = false, bool PRED = false>
 __device__ __forceinline__ void quad_node(const SweepArgs &a, const int slot, const int d) {
     if constexpr (D16) {
         if (__builtin_amdgcn_readfirstlane(*a.fit_flag) == 0) {
-            quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, true>(a, slot, d);
+            quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, true, PRED>(a, slot, d);
             return;
         }
     }
-    quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, false>(a, slot, d);
+    quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, false, PRED>(a, slot, d);
 }
 
 }  // namespace

@@ -306,14 +306,14 @@ class Pipeline:
 # ---------------------------------------------------------------- training
 
 def train(wl: Workload, n_instances=20000, batch=16, lr=2e-3, reuse=2, seed_base=1_000_000, ckpt=None,
-          log=print, resume=None, time_budget_s=None, target_level=0):
+          log=print, resume=None, time_budget_s=None, target_level=0, hidden=32, layers=6):
     """Online training: each batch of fresh seeded instances is labelled by
     plain HBP on the GPU and used for `reuse` Adam steps.  The targets are
     plain HBP's outboxes of `target_level` (0: the converged finest level,
     averaged over 4x4 blocks; 2: the 1/4-resolution level at the end of its
     half-sweeps, the state that plain HBP hands to level 1)."""
     import torch
-    net = ConvNet.load(resume) if resume else ConvNet(wl.L)
+    net = ConvNet.load(resume) if resume else ConvNet(wl.L, hidden, layers)
     if resume:
         target_level = net.meta.get("target_level", 0)
     pipe = Pipeline(wl)
@@ -390,15 +390,19 @@ class WarmStart:
             self.plan.run_schedule(costs.data_ptr(), labels.data_ptr(), len(schedule) - 1, schedule, False,
                                    self.stream())
             return
-        if self.net is not None:
-            _ck(G.lib().ginf_minpool_costs(_vp(costs), G.F32, wl.W, wl.H, wl.L, POOL, IN_SCALE, _vp(self.xin),
-                                          ctypes.c_void_p(self.stream())))
-            self.net.infer(self.xin, 1, wl.h4, wl.w4, self.pred)
-        else:
-            self.pred.zero_()
-        shift = 2 - seed_level
-        self.plan.seed(seed_level, self.pred.data_ptr(), wl.w4, wl.h4, shift, wl.msg_scale, self.stream())
-        self.plan.run_schedule(costs.data_ptr(), labels.data_ptr(), seed_level, schedule, True, self.stream())
+        if self.net is None:
+            # zero messages at the seeded level: an unseeded start there (the
+            # same result as seeding zeros, without storing or reading them)
+            self.plan.run_schedule(costs.data_ptr(), labels.data_ptr(), seed_level, schedule, False, self.stream())
+            return
+        # One pass over the costs builds the level data and the network input
+        # (ginf_bp_plan_prepare); the network runs in one launch; the first
+        # half-sweep of the seeded level reads the prediction itself (same
+        # result as ginf_bp_plan_seed followed by a seeded schedule).
+        self.plan.prepare(costs.data_ptr(), seed_level, self.xin.data_ptr(), IN_SCALE, self.stream())
+        self.net.infer(self.xin, 1, wl.h4, wl.w4, self.pred)
+        self.plan.run_predicted(costs.data_ptr(), labels.data_ptr(), seed_level, schedule, self.pred.data_ptr(), wl.w4,
+                                wl.h4, 2 - seed_level, wl.msg_scale, self.stream(), prepared=True)
 
 
 def default_model_path(wl: Workload) -> str:
@@ -423,7 +427,7 @@ def _timed(torch, fn, reps=5):
 # (seed level, half-sweeps per level from the seeded level down to level 0);
 # seed level -1 is plain HBP from zero messages on all 5 levels (a baseline
 # with fewer iterations than the reference's 10 per level).
-SCHEDULES = [(0, (4,)), (0, (6,)), (0, (8,)), (0, (10,)), (0, (12,)), (0, (14,)),
+SCHEDULES = [(0, (4,)), (0, (6,)), (0, (8,)), (0, (10,)), (0, (11,)), (0, (12,)), (0, (13,)), (0, (14,)),
              (1, (2, 4)), (1, (4, 4)), (1, (4, 6)), (1, (6, 6)), (1, (6, 8)), (1, (0, 10)), (1, (2, 10)), (1, (3, 10)),
              (1, (4, 10)), (1, (6, 10)), (1, (8, 10)), (1, (10, 10)), (1, (4, 8)), (1, (6, 9)),
              (2, (4, 4, 6)), (2, (6, 6, 6)), (2, (6, 6, 8)), (2, (0, 8, 10)), (2, (0, 10, 8)), (2, (0, 10, 10)),
@@ -495,6 +499,8 @@ def main(argv=None):
     ap.add_argument("--budget", type=float, default=None, help="training time budget in seconds")
     ap.add_argument("--target-level", type=int, default=0, choices=[0, 1, 2],
                     help="train: pyramid level whose plain-HBP outboxes are the targets")
+    ap.add_argument("--hidden", type=int, default=32, help="train: hidden channels (a multiple of 32)")
+    ap.add_argument("--layers", type=int, default=6, help="train: convolution layers")
     ap.add_argument("--eval-seeds", type=int, default=8)
     ap.add_argument("--out", default=None, help="eval: write JSON records here")
     a = ap.parse_args(argv)
@@ -502,7 +508,7 @@ def main(argv=None):
     path = a.model or default_model_path(wl)
     if a.cmd == "train":
         net, hist = train(wl, a.instances, a.batch, a.lr, a.reuse, ckpt=path, resume=a.resume, time_budget_s=a.budget,
-                          target_level=a.target_level)
+                          target_level=a.target_level, hidden=a.hidden, layers=a.layers)
         print(json.dumps({"model": path, "params": net.params(), "instances": hist[-1][0], "final_loss": hist[-1][1],
                           "first_loss": hist[0][1]}))
     else:

@@ -194,6 +194,32 @@ int ginf_bp_plan_run_schedule(ginf_bp_plan plan, const void *data_dev, int32_t *
 int ginf_bp_plan_seed(ginf_bp_plan plan, int level, const float *pred_dev, int pw, int ph, int shift, float scale,
                       void *stream);
 
+/* Level data for a scheduled run, built ahead of it (no reference
+ * counterpart): the data-cost pyramid of levels 0..start_level from data_dev,
+ * as the runs build it.  With minpool_out non-NULL the same pass over the
+ * costs also writes the warm-start network's input: the costs min-pooled over
+ * 4 x 4 pixel blocks, times minpool_scale, rounded to BF16 (uint16_t), NHWC
+ * [(H+3)/4][(W+3)/4][L] -- the values of ginf_minpool_costs with f = 4. */
+int ginf_bp_plan_prepare(ginf_bp_plan plan, const void *data_dev, int start_level, uint16_t *minpool_out,
+                         float minpool_scale, void *stream);
+
+/* ginf_bp_plan_run_predicted flag: use the level data of the last
+ * ginf_bp_plan_prepare (on the same data) instead of building it again. */
+#define GINF_RUN_PREPARED 1
+
+/* Seeded scheduled run without the seeding pass (no reference counterpart):
+ * the same results as ginf_bp_plan_seed(plan, start_level, pred_dev, pw, ph,
+ * shift, scale) followed by ginf_bp_plan_run_schedule(..., seeded = 1), but
+ * where the label-quad kernel runs the level (L % 4 == 0, a 16-byte aligned
+ * prediction, iters[start_level] >= 2) the first half-sweep reads its
+ * incoming messages from pred_dev itself and no outbox of the seeded level
+ * is written before it; other plans fall back to seeding first.  pred_dev
+ * must stay valid until the run has executed on `stream`.  flags: 0 or
+ * GINF_RUN_PREPARED. */
+int ginf_bp_plan_run_predicted(ginf_bp_plan plan, const void *data_dev, int32_t *labels_dev, int start_level,
+                               const int *iters, const float *pred_dev, int pw, int ph, int shift, float scale,
+                               int flags, void *stream);
+
 /* Learned warm start building blocks (no reference counterpart): BF16 3x3
  * convolutions on MFMA with FP32 accumulation.  Tensors NHWC, weights
  * [Cout][3][3][Cin], bf16 passed as uint16_t.  Cin % 32 == 0, Cout % 32 == 0. */

@@ -92,16 +92,22 @@ def test_conv_weight_gradient_shapes(N, H, W, Ci, Co):
     assert rel(db.cpu().numpy(), rdb) < 1e-5
 
 
-def test_infer_matches_forward():
-    """The one-call network forward (ginf_convnet_fwd) gives the per-layer
-    forward's output bit for bit."""
+@pytest.mark.parametrize("L,N,H,W", [(64, 2, 13, 21), (64, 1, 120, 160), (64, 4, 8, 10), (64, 1, 1, 1),
+                                     (96, 3, 45, 80), (96, 1, 7, 9), (96, 2, 180, 320), (32, 2, 11, 12)])
+def test_infer_matches_forward(L, N, H, W):
+    """The one-call network forward (ginf_convnet_fwd: the one-launch fused
+    kernel for 64 and 96 labels, layer by layer otherwise) gives the per-layer
+    forward's output bit for bit, on image sizes that are and are not
+    multiples of its 8 x 10 tile."""
     torch = torch_cuda()
     from gridinfcuda_amd.warmstart import ConvNet
-    say("one-call network forward")
-    net = ConvNet(64, seed=5)
+    say(f"one-call network forward L={L} N={N} H={H} W={W}")
+    net = ConvNet(L, seed=5)
     rng = np.random.default_rng(6)
-    N, H, W = 2, 13, 21
-    x = dev_bf16(torch, rng.standard_normal((N, H, W, 64)))
+    # non-zero biases so that the padding ring of every layer matters
+    for b in net.b:
+        b.copy_(torch.from_numpy(rng.standard_normal(b.shape).astype(np.float32) * 0.3))
+    x = dev_bf16(torch, rng.standard_normal((N, H, W, L)))
     a = net.forward(x, N, H, W).clone()
     b = torch.empty_like(a)
     net.infer(x, N, H, W, b)
@@ -185,6 +191,67 @@ def test_minpool_shapes(W, H, L, dt):
     np.testing.assert_array_equal(got, C.bf16_round(np.transpose(ref, (1, 2, 0)).astype(np.float32) * np.float32(IN_SCALE)))
 
 
+
+@pytest.mark.parametrize("W,H,L,dt,levels,start", [(64, 45, 32, "f32", 1, 0), (130, 90, 64, "f32", 3, 0),
+                                                   (70, 45, 32, "f32", 3, 2), (97, 43, 18, "f32", 2, 1),
+                                                   (96, 42, 24, "i32", 2, 0), (67, 41, 96, "i32", 3, 1),
+                                                   (50, 30, 300, "f32", 1, 0)])
+def test_prepare_minpool_matches(W, H, L, dt, levels, start):
+    """ginf_bp_plan_prepare writes the min-pooled network input in the same
+    pass as the level data (label-quad layout; the separate kernel for other
+    layouts): equal to ginf_minpool_costs, and a run on the prepared data
+    (GINF_RUN_PREPARED) equals a run that prepares itself."""
+    torch = torch_cuda()
+    from gridinfcuda_amd.warmstart import POOL, IN_SCALE
+    say(f"prepare + minpool W={W} H={H} L={L} {dt} levels={levels} start={start}")
+    rng = np.random.default_rng(W * 7 + H + L)
+    c = rng.integers(0, 41, size=(L, H, W))
+    if dt == "f32" and L == 64:
+        c = c + 0.5  # fractional costs: 32-bit data storage, the relayout's second pass
+    np_t = np.float32 if dt == "f32" else np.int32
+    costs = torch.from_numpy(c.astype(np_t)).cuda()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    w4, h4 = (W + 3) // 4, (H + 3) // 4
+    ref = torch.empty((h4, w4, L), dtype=torch.int16, device="cuda")
+    G.check(G.lib().ginf_minpool_costs(vp(costs), G.F32 if dt == "f32" else G.I32, W, H, L, POOL,
+                                       ctypes.c_float(IN_SCALE), vp(ref), s))
+    a = np.arange(L)
+    table = np.minimum(4 * np.abs(a[:, None] - a[None, :]), 12).astype(np_t)
+    norm = G.NORM_MIN if L > 256 else G.NORM_MEAN
+    plan = G.Plan(G.make_desc(W, H, L, G.F32 if dt == "f32" else G.I32, G.SM_TRUNC_LINEAR, levels, 3, norm), table)
+    got = torch.full((h4, w4, L), -1, dtype=torch.int16, device="cuda")
+    plan.prepare(costs.data_ptr(), start, got.data_ptr(), IN_SCALE, s.value)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(got.cpu().numpy(), ref.cpu().numpy())
+    lv = [plan.dump_data(t) for t in range(start + 1)]
+    if dt != "f32":
+        return  # seeded runs need FP32 messages
+    lab = [torch.zeros((H, W), dtype=torch.int32, device="cuda") for _ in range(2)]
+    f = 1 << start
+    shift = 2 - start
+    pw = ((W + f - 1) // f + (1 << shift) - 1) >> shift
+    ph = ((H + f - 1) // f + (1 << shift) - 1) >> shift
+    pred = torch.from_numpy(rng.standard_normal((ph, pw, 4 * L)).astype(np.float32)).cuda()
+    sched = [3] * (start + 1)
+    plan.run_predicted(costs.data_ptr(), lab[0].data_ptr(), start, sched, pred.data_ptr(), pw, ph, shift, 0.5,
+                       s.value, prepared=True)
+    torch.cuda.synchronize()
+    m0 = [plan.dump_msgs(t) for t in range(start + 1)]
+    for t in range(start + 1):
+        np.testing.assert_array_equal(plan.dump_data(t), lv[t])
+    plan.run_predicted(costs.data_ptr(), lab[1].data_ptr(), start, sched, pred.data_ptr(), pw, ph, shift, 0.5,
+                       s.value)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(lab[0].cpu().numpy(), lab[1].cpu().numpy())
+    for t in range(start + 1):
+        np.testing.assert_array_equal(plan.dump_msgs(t), m0[t])
+    # a level above the prepared one is refused
+    if start + 1 < levels:
+        with pytest.raises(G.GinfError):
+            plan.prepare(costs.data_ptr(), start, 0, 0.0, s.value)
+            plan.run_predicted(costs.data_ptr(), lab[0].data_ptr(), start + 1, sched + [3], pred.data_ptr(), pw, ph,
+                               shift, 0.5, s.value, prepared=True)
+
 def test_costs_pool_and_targets():
     torch = torch_cuda()
     from gridinfcuda_amd.warmstart import Workload, Pipeline, POOL, IN_SCALE
@@ -267,6 +334,61 @@ def test_seed_and_schedule(oracle_mod):
     np.testing.assert_array_equal(lab.cpu().numpy()[1:-1, 1:-1], want[1:-1, 1:-1])
     np.testing.assert_array_equal(plan.dump_msgs(0), msgs0)
 
+
+
+# (W, H, L, levels, start level, iters from the start level down to 0, integer costs)
+PRED_CASES = [
+    (300, 200, 16, 1, 0, (2,), True), (300, 200, 16, 1, 0, (3,), True), (300, 200, 16, 1, 0, (4,), True),
+    (300, 200, 16, 1, 0, (5,), True), (300, 200, 16, 1, 0, (6,), True), (300, 200, 16, 1, 0, (7,), True),
+    (300, 200, 16, 1, 0, (8,), True), (300, 200, 32, 2, 1, (6, 9), True), (97, 61, 12, 3, 2, (4, 3, 5), True),
+    (130, 90, 64, 3, 0, (5,), True), (130, 90, 64, 3, 1, (4, 6), False), (130, 90, 64, 3, 2, (2, 2, 3), True),
+    (83, 77, 96, 2, 0, (4,), False), (70, 50, 128, 2, 1, (3, 2), True),
+    # fallbacks to seeding first: L % 4 != 0, one half-sweep at the seeded level, chunked layout (min norm, int-valued)
+    (90, 70, 18, 1, 0, (4,), True), (90, 70, 16, 2, 1, (1, 4), True), (90, 70, 300, 1, 0, (3,), True),
+]
+
+
+@pytest.mark.parametrize("W,H,L,levels,start,iters,integer", PRED_CASES)
+def test_run_predicted_equals_seed_then_run(W, H, L, levels, start, iters, integer):
+    """ginf_bp_plan_run_predicted (the first half-sweep reads the prediction)
+    gives the labels and messages of ginf_bp_plan_seed + a seeded schedule."""
+    torch = torch_cuda()
+    say(f"run_predicted {W}x{H}x{L} levels={levels} start={start} iters={iters} integer={integer}")
+    rng = np.random.default_rng(W * 31 + L)
+    data = G.gen_stereo(11, W, H, L, 6, L - 1, 20)["cost_f32"]
+    if not integer:
+        data = data + rng.random(data.shape, dtype=np.float32) * 0.25
+    other = G.gen_stereo(12, W, H, L, 6, L - 1, 20)["cost_f32"]
+    a = np.arange(L)
+    table = np.minimum(5 * np.abs(a[:, None] - a[None, :]), 15).astype(np.float32)
+    norm = G.NORM_MIN if L > 256 else G.NORM_MEAN
+    desc = G.make_desc(W, H, L, G.F32, G.SM_TRUNC_LINEAR, levels, 5, norm)
+    shift = 2 - start
+    f = 1 << start
+    lw, lh = (W + f - 1) // f, (H + f - 1) // f
+    pw, ph = (lw + (1 << shift) - 1) >> shift, (lh + (1 << shift) - 1) >> shift
+    pred = torch.from_numpy(rng.standard_normal((ph, pw, 4 * L)).astype(np.float32)).cuda()
+    dd = torch.from_numpy(data).cuda()
+    do = torch.from_numpy(other).cuda()
+    s = torch.cuda.current_stream().cuda_stream
+    sched = list(reversed(iters))
+    outs = []
+    for direct in (False, True):
+        plan = G.Plan(desc, table)
+        lab = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+        plan.run(do.data_ptr(), lab.data_ptr(), s)  # leave another problem's messages in every array
+        lab.fill_(-7)
+        if direct:
+            plan.run_predicted(dd.data_ptr(), lab.data_ptr(), start, sched, pred.data_ptr(), pw, ph, shift, 0.5, s)
+        else:
+            plan.seed(start, pred.data_ptr(), pw, ph, shift, 0.5, s)
+            plan.run_schedule(dd.data_ptr(), lab.data_ptr(), start, sched, True, s)
+        torch.cuda.synchronize()
+        outs.append((lab.cpu().numpy(), [plan.dump_msgs(t) for t in range(start + 1)]))
+        plan.close()
+    np.testing.assert_array_equal(outs[1][0], outs[0][0])
+    for t in range(start + 1):
+        np.testing.assert_array_equal(outs[1][1][t], outs[0][1][t], err_msg=f"level {t}")
 
 def test_training_lowers_loss():
     torch = torch_cuda()
