@@ -33,7 +33,7 @@ EXPORTS = [
     "ginf_icm", "ginf_max_prior", "ginf_icm_host", "ginf_max_prior_host", "ginf_alpha_beta",
     "ginf_alpha_beta_host", "ginf_stream_probe", "ginf_bp_plan_set_compact_data", "ginf_bp_plan_compact_data",
     "ginf_batch_set_compact_data", "ginf_bp_plan_set_pairs", "ginf_bp_plan_pairs", "ginf_convnet_fwd",
-    "ginf_bp_plan_run_predicted", "ginf_bp_plan_prepare",
+    "ginf_bp_plan_run_predicted", "ginf_bp_plan_prepare", "ginf_batch_set_predictions",
 ]
 
 

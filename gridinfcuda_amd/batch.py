@@ -1,8 +1,9 @@
 """Batched BP (ginf_batch_*) and the seeded batched workloads.
 
-A batch advances many independent single-level MRFs together: every
-half-sweep of every instance is one persistent launch over a work list, and
-the whole decode is captured once into a HIP graph and replayed.
+A batch advances many independent single-level MRFs together: each
+half-sweep of every instance of one label capacity is one grid-stride launch
+over a work list, and the whole decode is captured once into a HIP graph and
+replayed.
 """
 from __future__ import annotations
 
@@ -31,9 +32,11 @@ def _declare():
     L.ginf_batch_set_seeded.argtypes = [VP, I]
     L.ginf_batch_seed.argtypes = [VP, I, VP, I, I, I, ctypes.c_float, VP]
     L.ginf_batch_set_compact_data.argtypes = [VP, I]
+    L.ginf_batch_set_predictions.argtypes = [VP, ctypes.POINTER(VP), ctypes.POINTER(I), ctypes.POINTER(I), I,
+                                             ctypes.POINTER(ctypes.c_float)]
     for n in ("ginf_batch_create", "ginf_batch_run", "ginf_batch_set_tiles", "ginf_batch_updates_per_run",
               "ginf_batch_dump_msgs", "ginf_batch_destroy", "ginf_batch_set_seeded", "ginf_batch_seed",
-              "ginf_batch_set_compact_data"):
+              "ginf_batch_set_compact_data", "ginf_batch_set_predictions"):
         getattr(L, n).restype = I
     _declared = True
 
@@ -72,6 +75,20 @@ class Batch:
     def seed(self, i: int, pred_ptr: int, pw: int, ph: int, shift: int, scale: float, stream: int = 0) -> None:
         G.check(G.lib().ginf_batch_seed(self._h, i, ctypes.c_void_p(pred_ptr), pw, ph, shift, scale,
                                         ctypes.c_void_p(stream)))
+
+    def set_predictions(self, preds, shift: int = 2) -> None:
+        """preds: one (pointer, pw, ph, scale) per instance, or None to switch
+        off: the first half-sweep of every run reads instance i's incoming
+        messages from its prediction (ginf_batch_set_predictions)."""
+        if preds is None:
+            G.check(G.lib().ginf_batch_set_predictions(self._h, None, None, None, 0, None))
+            return
+        n = self.n
+        ptr = (ctypes.c_void_p * n)(*[p[0] for p in preds])
+        pw = (ctypes.c_int * n)(*[p[1] for p in preds])
+        ph = (ctypes.c_int * n)(*[p[2] for p in preds])
+        sc = (ctypes.c_float * n)(*[p[3] for p in preds])
+        G.check(G.lib().ginf_batch_set_predictions(self._h, ptr, pw, ph, shift, sc))
 
     def updates_per_run(self) -> float:
         u = ctypes.c_double()

@@ -123,12 +123,14 @@ struct BatchItem {
     int nslots;
     int pad;
 };
+// pred: the first half-sweep of a predicted batch (args carry each instance's
+// prediction as the incoming-message source; FP32 only).
 hipError_t launch_sweep_batch(int dtype, int lmax, bool full, const SweepArgs *args_dev, const BatchItem *items_dev, int nitems,
-                              int grid, bool zeroin, bool labels, hipStream_t st);
+                              int grid, bool zeroin, bool labels, hipStream_t st, bool pred = false);
 // The same with each instance's data costs in 16-bit storage when its
 // SweepArgs::fit_flag reads 0 (sweep_batch16.hip).
 hipError_t launch_sweep_batch16(int dtype, int lmax, bool full, const SweepArgs *args_dev, const BatchItem *items_dev, int nitems,
-                                int grid, bool zeroin, bool labels, hipStream_t st);
+                                int grid, bool zeroin, bool labels, hipStream_t st, bool pred = false);
 
 // Seed a level's outboxes from an NHWC prediction (conv_net.hip).
 hipError_t launch_inject(const float *pred, void *const msgs[2][4], const LevelGeom &g, int C, int G, bool interleave,

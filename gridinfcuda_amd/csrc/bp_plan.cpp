@@ -1104,6 +1104,11 @@ struct ginf_batch_s {
     int blocks_per_cu = 4;
     int num_cu = 0;
     bool seeded = false;  // first half-sweep reads the outboxes stored by ginf_batch_seed
+    // ginf_batch_set_predictions: the first half-sweep reads each instance's
+    // prediction (args of that half-sweep in pred_args_dev)
+    bool predicted = false;
+    std::vector<ginf::SweepArgs> pred_args;
+    ginf::SweepArgs *pred_args_dev = nullptr;
     bool compact = true;  // 16-bit data costs per instance where they fit (each plan's fit flag)
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1192,6 +1197,11 @@ int batch_capture(ginf_batch b, const void *const *data_dev, int32_t *const *lab
         }
     GINF_HIP_TRY(hipMemcpy(b->args_dev, b->args_host[0].data(), sizeof(SweepArgs) * b->n, hipMemcpyHostToDevice));
     GINF_HIP_TRY(hipMemcpy(b->args_dev + b->n, b->args_host[1].data(), sizeof(SweepArgs) * b->n, hipMemcpyHostToDevice));
+    if (b->predicted) {
+        // half-sweep 0 moves colour 1: its arguments with the prediction as the source
+        for (int i = 0; i < b->n; ++i) b->pred_args[i].labels = labels_dev[i];
+        GINF_HIP_TRY(hipMemcpy(b->pred_args_dev, b->pred_args.data(), sizeof(SweepArgs) * b->n, hipMemcpyHostToDevice));
+    }
     const int N = b->iters;
     hipStream_t st = b->cap;
     for (int k = 0; k < b->ngroups; ++k) {  // branch streams, created on first use
@@ -1246,11 +1256,13 @@ int batch_capture(ginf_batch b, const void *const *data_dev, int32_t *const *lab
         hipStream_t gs = b->group_stream[k];
         for (int it = 0; it < N; ++it) {
             const int color = (it + 1) & 1;
-            const bool zeroin = N >= 2 && it == 0 && !b->seeded;
+            const bool pred = b->predicted && it == 0;
+            const bool zeroin = N >= 2 && it == 0 && !b->seeded && !b->predicted;
             const bool labels = it >= N - 2;
             e = (b->compact ? launch_sweep_batch16 : launch_sweep_batch)(
-                b->dtype, b->group_lmax[k], b->group_full[k], b->args_dev + (size_t)color * b->n,
-                b->items_dev + b->group_off[k], b->group_count[k], grid, zeroin, labels, gs);
+                b->dtype, b->group_lmax[k], b->group_full[k],
+                pred ? b->pred_args_dev : b->args_dev + (size_t)color * b->n, b->items_dev + b->group_off[k],
+                b->group_count[k], grid, zeroin, labels, gs, pred);
             if (e != hipSuccess) return fail(e);
         }
         if ((e = hipEventRecord(b->join_ev[k], gs)) != hipSuccess) return fail(e);
@@ -1386,6 +1398,45 @@ int ginf_batch_set_seeded(ginf_batch b, int seeded) {
     return GINF_OK;
 }
 
+int ginf_batch_set_predictions(ginf_batch b, const float *const *pred_dev, const int *pw, const int *ph, int shift,
+                               const float *scale) {
+    if (!b) return set_error(GINF_ERR_INVALID, "NULL batch");
+    if (!pred_dev) {
+        if (b->predicted) batch_drop_graph(b);
+        b->predicted = false;
+        return GINF_OK;
+    }
+    if (!pw || !ph || !scale || shift < 0 || shift > 8) return set_error(GINF_ERR_INVALID, "bad prediction arguments");
+    if (b->dtype != GINF_F32) return set_error(GINF_ERR_UNSUPPORTED, "predictions need FP32 messages");
+    if (b->iters < 2) return set_error(GINF_ERR_UNSUPPORTED, "predictions need two half-sweeps or more");
+    std::vector<ginf::SweepArgs> pa((size_t)b->n);
+    for (int i = 0; i < b->n; ++i) {
+        const int L = b->plans[i]->desc.num_labels;
+        if (!pred_dev[i] || pw[i] < 1 || ph[i] < 1 || scale[i] == 0.f)
+            return set_error(GINF_ERR_INVALID, "bad prediction for instance %d", i);
+        if (L % 4 || ((uintptr_t)pred_dev[i] & 15))
+            return set_error(GINF_ERR_UNSUPPORTED, "instance %d: predictions need L %% 4 == 0 and 16-byte alignment", i);
+        const size_t bytes = (size_t)pw[i] * ph[i] * 4 * L * sizeof(float);
+        if (bytes >= ((size_t)1 << 32)) return set_error(GINF_ERR_UNSUPPORTED, "instance %d: prediction too large", i);
+        ginf::SweepArgs a = b->args_host[1][i];  // half-sweep 0 moves colour 1
+        a.in_base = pred_dev[i];
+        a.in_span = (uint32_t)bytes;
+        a.in_plane_bytes = 16;
+        a.pred_w = pw[i];
+        a.pred_h = ph[i];
+        a.pred_shift = shift;
+        a.pred_inv_scale = 1.f / scale[i];
+        pa[(size_t)i] = a;
+    }
+    if (!b->pred_args_dev) GINF_HIP_TRY(hipMalloc(&b->pred_args_dev, sizeof(ginf::SweepArgs) * b->n));
+    b->pred_args = pa;
+    // a captured graph reads the arguments from pred_args_dev at every replay;
+    // a new mode needs another kernel instance, so the graph is captured again
+    batch_drop_graph(b);
+    b->predicted = true;
+    return GINF_OK;
+}
+
 int ginf_batch_seed(ginf_batch b, int instance, const float *pred_dev, int pw, int ph, int shift, float scale,
                     void *stream) {
     if (!b || instance < 0 || instance >= b->n) return set_error(GINF_ERR_INVALID, "bad instance");
@@ -1419,6 +1470,7 @@ int ginf_batch_destroy(ginf_batch b) {
     }
     if (b->fork_ev) hipEventDestroy(b->fork_ev);
     if (b->args_dev) hipFree(b->args_dev);
+    if (b->pred_args_dev) hipFree(b->pred_args_dev);
     if (b->items_dev) hipFree(b->items_dev);
     for (auto p : b->plans) ginf_bp_plan_destroy(p);
     delete b;

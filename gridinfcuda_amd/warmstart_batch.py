@@ -9,10 +9,11 @@ One pass over a chunk of instances:
   1. device costs from the image pairs (ginf_stereo_costs);
   2. per class, the cost volumes min-pooled to 1/4 resolution and one
      batched network forward on MFMA (ginf_conv3x3_fwd);
-  3. each instance's finest-level outboxes seeded from its prediction
-     (ginf_batch_seed, 4x outbox upsample);
-  4. one seeded batched finest-level BP of K half-sweeps over all instances
-     (ginf_batch_run: persistent work-list kernel, HIP graph replay);
+  3-4. one batched finest-level BP of K half-sweeps over all instances
+     (ginf_batch_run: grid-stride work-list launches, HIP graph replay) whose
+     first half-sweep reads each instance's prediction as its incoming
+     messages (ginf_batch_set_predictions; 4x outbox upsample, the same
+     results as seeding the outboxes with ginf_batch_seed);
   5. FP64 energies of the labellings (ginf_energy_f64).
 Plain hierarchical BP (5 levels x 10 half-sweeps, one plan per class) is the
 energy reference: an instance whose warm-started energy exceeds the plain energy by more
@@ -97,7 +98,15 @@ class BatchedWarmStart:
         descs = [G.make_desc(classes[c].W, classes[c].H, classes[c].L, G.F32, classes[c].model, 1, iters, G.NORM_MEAN)
                  for c in self.cc]
         self.batch = B.Batch(descs, [classes[c].table for c in self.cc])
-        self.batch.set_seeded(True)
+        # the batch's first half-sweep reads each instance's prediction
+        # (ginf_batch_set_predictions: no seeding pass); zero predictions are
+        # the ablation
+        preds = []
+        for k, c in enumerate(self.cc):
+            wl = classes[c]
+            j = self.members[c].index(k)
+            preds.append((self.pred[c][j].data_ptr(), wl.w4, wl.h4, wl.msg_scale))
+        self.batch.set_predictions(preds, shift=2)
         self.tiles_cached = autotune.apply_cached(self.batch)
 
     def stream(self):
@@ -118,7 +127,8 @@ class BatchedWarmStart:
                                              _vp(self.costs[k]), ctypes.c_void_p(self.stream())))
 
     def warm(self, use_net: bool = True):
-        """Steps 2-4: network inference per class, seeding, one batched BP."""
+        """Steps 2-4: network inference per class into the predictions the
+        batch reads, then one batched BP (its first half-sweep reads them)."""
         st = self.stream()
         for c, idx in self.members.items():
             wl = self.classes[c]
@@ -129,8 +139,6 @@ class BatchedWarmStart:
                 self.nets[c].infer(self.xin[c], len(idx), wl.h4, wl.w4, self.pred[c])
             else:
                 self.pred[c].zero_()
-            for j, k in enumerate(idx):
-                self.batch.seed(k, self.pred[c][j].data_ptr(), wl.w4, wl.h4, 2, wl.msg_scale, st)
         self.batch.run([t.data_ptr() for t in self.costs], [t.data_ptr() for t in self.labels], st)
 
     def plain(self):

@@ -285,6 +285,17 @@ int ginf_batch_set_seeded(ginf_batch batch, int seeded);
 int ginf_batch_set_compact_data(ginf_batch batch, int enable);
 int ginf_batch_seed(ginf_batch batch, int instance, const float *pred_dev, int pw, int ph, int shift, float scale,
                     void *stream);
+/* Predictions as the first half-sweep's incoming messages (no reference
+ * counterpart): instance i's outboxes are read from pred_dev[i] (FP32 NHWC
+ * [ph[i]][pw[i]][4L], 1/2^shift of the instance's resolution, divided by
+ * scale[i], border neighbours 0) -- the messages ginf_batch_seed would store,
+ * without storing them.  Same results as seeding every instance and running
+ * with ginf_batch_set_seeded(batch, 1).  The pointers are read at every
+ * ginf_batch_run until the next call; pred_dev == NULL switches it off.
+ * FP32 batches with two half-sweeps or more, L % 4 == 0, 16-byte aligned
+ * predictions. */
+int ginf_batch_set_predictions(ginf_batch batch, const float *const *pred_dev, const int *pw, const int *ph, int shift,
+                               const float *scale);
 int ginf_batch_dump_msgs(ginf_batch batch, int instance, void *msgs_host);
 int ginf_batch_destroy(ginf_batch batch);
 

@@ -136,6 +136,63 @@ def test_seeded_batch(oracle_mod):
         np.testing.assert_array_equal(bt.dump_msgs(k), m, err_msg=f"inst {k}")
 
 
+
+@pytest.mark.parametrize("iters,compact", [(2, True), (3, True), (6, True), (4, False)])
+def test_batch_predictions_equal_seeding(iters, compact):
+    """ginf_batch_set_predictions (the first half-sweep reads each instance's
+    prediction) gives the labels and messages of seeding every instance with
+    ginf_batch_seed and running seeded, for every label capacity (16-128,
+    full and partial), both data-cost forms and repeated replays; switching
+    the predictions off returns the unseeded batch."""
+    torch = torch_cuda()
+    say(f"batch predictions iters={iters} compact={compact}")
+    insts = [x for x in small_instances(16, 41) if x["W"] >= 4 and x["H"] >= 4 and x["L"] % 4 == 0]
+    assert {x["L"] for x in insts} >= {16, 32, 64, 96, 128}
+    datas = [B.instance_costs(x) for x in insts]
+    if not compact:
+        datas = [d + 0.25 for d in datas]  # fractional costs: 32-bit storage
+    inits = [np.zeros((x["H"], x["W"]), np.int32) for x in insts]
+    s = torch.cuda.current_stream().cuda_stream
+    rng = np.random.default_rng(iters)
+    preds = []
+    for x in insts:
+        pw, ph = (x["W"] + 3) // 4, (x["H"] + 3) // 4
+        preds.append(torch.from_numpy(rng.standard_normal((ph, pw, 4 * x["L"])).astype(np.float32) * 8).cuda())
+    ref = B.make_batch(insts, iters=iters)
+    _, ll_plain = run_batch(torch, ref, datas, inits)
+    plain_msgs = [ref.dump_msgs(k) for k in range(len(insts))]
+    ref.set_seeded(True)
+    for k in range(len(insts)):
+        ref.seed(k, preds[k].data_ptr(), preds[k].shape[1], preds[k].shape[0], 2, 0.5, s)
+    _, ll_ref = run_batch(torch, ref, datas, inits)
+    ref_msgs = [ref.dump_msgs(k) for k in range(len(insts))]
+    bt = B.make_batch(insts, iters=iters)
+    bt.set_predictions([(preds[k].data_ptr(), preds[k].shape[1], preds[k].shape[0], 0.5) for k in range(len(insts))])
+    for rep in range(2):
+        _, ll = run_batch(torch, bt, datas, inits)
+        for k in range(len(insts)):
+            np.testing.assert_array_equal(ll[k].cpu().numpy(), ll_ref[k].cpu().numpy(), err_msg=f"inst {k} rep {rep}")
+            np.testing.assert_array_equal(bt.dump_msgs(k), ref_msgs[k], err_msg=f"inst {k} rep {rep}")
+    bt.set_predictions(None)
+    _, ll = run_batch(torch, bt, datas, inits)
+    for k in range(len(insts)):
+        np.testing.assert_array_equal(ll[k].cpu().numpy(), ll_plain[k].cpu().numpy(), err_msg=f"off: inst {k}")
+        np.testing.assert_array_equal(bt.dump_msgs(k), plain_msgs[k], err_msg=f"off: inst {k}")
+
+
+def test_batch_predictions_refused():
+    """L % 4 != 0, one half-sweep, or int messages: refused with an error."""
+    torch = torch_cuda()
+    insts = [x for x in small_instances(16, 41) if x["W"] >= 4 and x["H"] >= 4]
+    odd = [x for x in insts if x["L"] % 4][:1] + [x for x in insts if x["L"] % 4 == 0][:1]
+    assert len(odd) == 2
+    p = torch.zeros(1 << 16, dtype=torch.float32, device="cuda")
+    for case in ("odd", "iters", "int"):
+        xs = odd if case == "odd" else odd[1:]
+        bt = B.make_batch(xs, iters=1 if case == "iters" else 3, dtype=np.int32 if case == "int" else np.float32)
+        with pytest.raises(G.GinfError):
+            bt.set_predictions([(p.data_ptr(), 2, 2, 1.0) for _ in xs])
+
 @pytest.mark.parametrize("compact", [True, False])
 def test_batch_data_storage_forms(oracle_mod, compact):
     """Each instance keeps 16-bit data costs when they fit and falls back on

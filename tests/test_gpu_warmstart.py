@@ -443,9 +443,10 @@ def test_trained_net_needs_fewer_halfsweeps_than_zero_seed():
 
 
 def test_batched_warm_start_pipeline():
-    """Batched inference + seeded batched BP: every instance's labels equal a
-    single-plan seeded run from the same prediction, and the fallback takes
-    the plain HBP labels exactly when the energy rule says so."""
+    """Batched inference + predicted batched BP (the first half-sweep reads
+    the predictions): every instance's labels equal a single-plan seeded run
+    from the same prediction, and the fallback takes the plain HBP labels
+    exactly when the energy rule says so."""
     torch = torch_cuda()
     from gridinfcuda_amd.warmstart import ConvNet, Workload
     from gridinfcuda_amd.warmstart_batch import BatchedWarmStart, fallback_flags
@@ -482,12 +483,10 @@ def test_batched_warm_start_pipeline():
     # A deliberately bad prediction (every seed message scaled far out of
     # range) must fall back: each flagged instance then carries plain HBP's
     # labels and energy exactly.
-    for c, idx in bws.members.items():
+    for c, idx in bws.members.items():  # the batch's first half-sweep reads these predictions
         wl = classes[c]
         bws.nets[c].forward(bws.xin[c], len(idx), wl.h4, wl.w4, out=bws.pred[c])
         bws.pred[c].mul_(-1e4)
-        for j, k in enumerate(idx):
-            bws.batch.seed(k, bws.pred[c][j].data_ptr(), wl.w4, wl.h4, 2, wl.msg_scale, s)
     bws.batch.run([t.data_ptr() for t in bws.costs], [t.data_ptr() for t in bws.labels], s)
     torch.cuda.synchronize()
     e_bad = bws.energies(bws.labels)
