@@ -127,7 +127,7 @@ def _interior_updates(W: int, H: int, L: int, iters: int) -> float:
 def bench_batched(args) -> dict:
     """Batched plain BP: 64 instances (widths log-uniform 160..1280, heights
     120..720, labels from {16,32,64,128}, Potts w=15 / TL lambda=8 tau=4
-    alternating), 20 half-sweeps each, one persistent launch per half-sweep,
+    alternating), 20 half-sweeps each, one grid-stride work-list launch per half-sweep and capacity,
     the whole decode replayed as one HIP graph; tiles from the autotuner."""
     import torch
     from gridinfcuda_amd import _lib as G

@@ -2,7 +2,7 @@
 src/include/bp.h:8-9).
 
 For a batch it searches, per label-capacity class (16/32/64/128), the number
-of slots in one work item, and the number of persistent blocks per compute
+of slots in one work item, and the number of grid-stride blocks per compute
 unit, timing whole graph replays on the GPU with events on the launch
 stream.  Every candidate computes the same results (the tile only changes
 how work is split), so the search is free to pick the fastest.  Results are

@@ -1077,7 +1077,7 @@ extern "C" int ginf_energy_f64_noalloc(int width, int height, int num_labels, in
 
 // ---------------------------------------------------------------------------
 // Batches: many single-level instances advanced together.  The work list is
-// grouped by label capacity; each group's half-sweeps are persistent launches
+// grouped by label capacity; each group's half-sweeps are grid-stride launches
 // of the kernel built for that capacity, on a graph branch of their own (the
 // groups share no state, so they run concurrently and only join at the end).
 // The whole decode (relayouts + all half-sweeps) is captured once into a HIP

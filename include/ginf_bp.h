@@ -265,15 +265,15 @@ int ginf_bp_plan_block_mean_msgs(ginf_bp_plan plan, int level, int f, float scal
 /* Batches (no reference counterpart; the batched workload of the benchmark
  * configurations): n independent single-level MRFs of any sizes and label
  * counts (Potts / truncated linear, L <= 128) sharing dtype, normalisation
- * and iteration count.  Each half-sweep of all instances is one persistent
- * launch over a work list; the whole decode is captured once into a HIP
+ * and iteration count.  Each half-sweep of all instances of one label
+ * capacity is one grid-stride launch over a work list; the whole decode is captured once into a HIP
  * graph (re-captured when the data/label pointers change) and replayed.
  * Results equal n separate ginf_bp_plan_run calls with GINF_KERNEL_QUAD. */
 typedef struct ginf_batch_s *ginf_batch;
 int ginf_batch_create(int n, const ginf_bp_desc *descs, const void *const *tables_host, ginf_batch *out);
 int ginf_batch_run(ginf_batch batch, const void *const *data_dev, int32_t *const *labels_dev, void *stream);
 /* Tile shape: slots per work item for label capacities 16, 32, 64, 128
- * (multiples of 64) and persistent blocks per compute unit. */
+ * (multiples of 64) and grid-stride blocks per compute unit. */
 int ginf_batch_set_tiles(ginf_batch batch, const int *slots_per_item, int blocks_per_cu);
 int ginf_batch_updates_per_run(ginf_batch batch, double *node_label_updates);
 /* Seeded batches: with seeded != 0 every instance starts from the outboxes
