@@ -337,10 +337,14 @@ def test_seed_and_schedule(oracle_mod):
 
 
 # (W, H, L, levels, start level, iters from the start level down to 0, integer costs)
+# 640x480 at 16 and 32 labels has enough tiles for the fused pairs of
+# half-sweeps (on by default there), so every arrangement of unfused first
+# half-sweeps and pairs at the predicted level is covered (2-9 half-sweeps).
 PRED_CASES = [
-    (300, 200, 16, 1, 0, (2,), True), (300, 200, 16, 1, 0, (3,), True), (300, 200, 16, 1, 0, (4,), True),
-    (300, 200, 16, 1, 0, (5,), True), (300, 200, 16, 1, 0, (6,), True), (300, 200, 16, 1, 0, (7,), True),
-    (300, 200, 16, 1, 0, (8,), True), (300, 200, 32, 2, 1, (6, 9), True), (97, 61, 12, 3, 2, (4, 3, 5), True),
+    (640, 480, 16, 1, 0, (2,), True), (640, 480, 16, 1, 0, (3,), True), (640, 480, 16, 1, 0, (4,), True),
+    (640, 480, 16, 1, 0, (5,), True), (640, 480, 16, 1, 0, (6,), True), (640, 480, 16, 1, 0, (7,), True),
+    (640, 480, 16, 1, 0, (8,), True), (640, 480, 16, 1, 0, (9,), False), (640, 480, 32, 2, 1, (6, 9), True),
+    (640, 480, 32, 2, 0, (7,), False), (300, 200, 16, 1, 0, (5,), True), (97, 61, 12, 3, 2, (4, 3, 5), True),
     (130, 90, 64, 3, 0, (5,), True), (130, 90, 64, 3, 1, (4, 6), False), (130, 90, 64, 3, 2, (2, 2, 3), True),
     (83, 77, 96, 2, 0, (4,), False), (70, 50, 128, 2, 1, (3, 2), True),
     # fallbacks to seeding first: L % 4 != 0, one half-sweep at the seeded level, chunked layout (min norm, int-valued)
@@ -375,6 +379,8 @@ def test_run_predicted_equals_seed_then_run(W, H, L, levels, start, iters, integ
     outs = []
     for direct in (False, True):
         plan = G.Plan(desc, table)
+        if W == 640:
+            assert plan.pairs()  # the fused pairs run at these sizes
         lab = torch.zeros((H, W), dtype=torch.int32, device="cuda")
         plan.run(do.data_ptr(), lab.data_ptr(), s)  # leave another problem's messages in every array
         lab.fill_(-7)
