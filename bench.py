@@ -570,7 +570,9 @@ def main() -> int:
     ap.add_argument("--pairs", type=int, default=None, choices=[0, 1],
                     help="fused pairs of half-sweeps: 1 on, 0 off (default: the plan's choice, on up to 32 labels)")
     ap.add_argument("--data32", action="store_true",
-                    help="keep 32-bit data costs (no 16-bit storage even when the costs fit)")
+                    help="keep 32-bit data costs (no 16- or 8-bit storage even when the costs fit)")
+    ap.add_argument("--data16", action="store_true",
+                    help="at most 16-bit data costs (no 8-bit storage of levels 0 and 1)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the all-cores CPU figure")
     ap.add_argument("--workload", default="hbp", choices=["hbp", "batched", "warmstart", "warmstart-batch", "config1",
                                                           "icm", "alpha-beta", "stereo"],
@@ -631,6 +633,8 @@ def main() -> int:
     plan = G.Plan(desc, table)
     if args.data32:
         plan.set_compact_data(False)
+    elif args.data16:
+        plan.set_compact_data(2)
     if args.pairs is not None:
         plan.set_pairs(bool(args.pairs))
     data = torch.from_numpy(gen).cuda()
@@ -662,6 +666,7 @@ def main() -> int:
         plan.run(data.data_ptr(), labels.data_ptr(), sp)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    level_updates = plan.sweep_level_updates()
     sweep_ms, launches, sweep_updates = plan.sweep_stats(reset=True)
     plan.set_sweep_timing(False)
 
@@ -670,11 +675,15 @@ def main() -> int:
     value = U / (ms_per_step / 1000.0) / 1e9
 
     # With 16-bit data costs (integer costs in [0, 65535], the plan's default for
-    # the quad kernel) an update reads 2 B of data cost instead of 4.
-    compact = plan.compact_data()
-    bytes_per_update = BYTES_PER_UPDATE - (2 if compact else 0)
+    # the quad kernel) an update reads 2 B of data cost instead of 4, and with
+    # 8-bit costs (levels 0 and 1, integers in [0, 255]) 1 B.
+    form = plan.data_form()
+    compact = form != 0
+    data_bytes = [4 if form == 0 else 1 if (form == 2 and t <= 1) else 2 for t in range(args.levels)]
+    alg_bytes = sum((BYTES_PER_UPDATE - 4 + db) * u for db, u in zip(data_bytes, level_updates))
+    bytes_per_update = alg_bytes / max(1.0, sweep_updates)
     avg_launch_ms = sweep_ms / max(1, launches)
-    bytes_per_launch = bytes_per_update * sweep_updates / max(1, launches)
+    bytes_per_launch = alg_bytes / max(1, launches)
     achieved = bytes_per_launch / (avg_launch_ms / 1000.0) / 1e9
     quad = args.kernel != 1
     pairs = plan.pairs()
@@ -684,7 +693,8 @@ def main() -> int:
     # HBM bytes per steady half-sweep launch, measured with rocprofv3 FETCH_SIZE /
     # WRITE_SIZE (scaled by a 1 GiB copy) on this workload; see the profile summary.
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"{'sweep_quad16' if compact else 'sweep_quad'}_1920x1080x128_traffic.json")
+    tname = ["sweep_quad", "sweep_quad16", "sweep_quad16_u8"][form]
+    tpath = os.path.join(ROOT, "profiles", f"{tname}_1920x1080x128_traffic.json")
     default_shape = (W, H, L, args.levels, args.iters, args.norm, args.kernel, args.lanes) == (1920, 1080, 128, 5, 10, 0, 0, 0) \
         and not pairs
     if default_shape and os.path.exists(tpath):
@@ -724,8 +734,8 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "kernel": f"{kname} steady-state instance (half-sweep)",
-            "bytes_per_update": bytes_per_update,
-            "data_costs": "16-bit" if compact else "32-bit",
+            "bytes_per_update": round(bytes_per_update, 4),
+            "data_costs": ["32-bit", "16-bit", "8-bit at levels 0-1, 16-bit above"][form],
             "avg_launch_ms": round(avg_launch_ms, 5),
             "launches_timed": launches,
             "algorithmic_bytes_per_launch": bytes_per_launch,

@@ -24,7 +24,7 @@ EXPORTS = [
     "ginf_bp_plan_create", "ginf_bp_plan_run", "ginf_bp_plan_destroy", "ginf_bp_plan_workspace_bytes",
     "ginf_hbp_host", "ginf_energy_f64", "ginf_energy_f64_noalloc", "ginf_bp_plan_dump_msgs", "ginf_bp_plan_level_dims",
     "ginf_gen_stereo", "ginf_last_error", "ginf_version", "ginf_bp_plan_set_sweep_timing",
-    "ginf_bp_plan_sweep_stats", "ginf_bp_plan_updates_per_run", "ginf_bp_plan_dump_data",
+    "ginf_bp_plan_sweep_stats", "ginf_bp_plan_sweep_level_updates", "ginf_bp_plan_updates_per_run", "ginf_bp_plan_dump_data",
     "ginf_batch_create", "ginf_batch_run", "ginf_batch_set_tiles", "ginf_batch_updates_per_run",
     "ginf_batch_dump_msgs", "ginf_batch_destroy", "ginf_bp_plan_run_schedule", "ginf_bp_plan_seed",
     "ginf_bp_plan_block_mean_msgs", "ginf_conv3x3_fwd", "ginf_conv3x3_bwd_weight", "ginf_conv3x3_flip_weights",
@@ -84,6 +84,7 @@ def lib() -> ctypes.CDLL:
         L.ginf_gen_stereo.argtypes = [ctypes.c_uint64, I, I, I, I, I, I, VP, VP, VP, VP, VP]
         L.ginf_bp_plan_set_sweep_timing.argtypes = [VP, I]
         L.ginf_bp_plan_sweep_stats.argtypes = [VP, P(ctypes.c_double), P(ctypes.c_int64), P(ctypes.c_double), I]
+        L.ginf_bp_plan_sweep_level_updates.argtypes = [VP, P(ctypes.c_double)]
         L.ginf_bp_plan_updates_per_run.argtypes = [VP, P(ctypes.c_double)]
         I64, F = ctypes.c_int64, ctypes.c_float
         L.ginf_bp_plan_run_schedule.argtypes = [VP, VP, VP, I, P(I), I, VP]
@@ -209,8 +210,10 @@ class Plan:
         check(lib().ginf_bp_plan_block_mean_msgs(self._h, level, f, scale, ctypes.c_void_p(out_ptr),
                                                  ctypes.c_void_p(stream)))
 
-    def set_compact_data(self, enable: bool) -> None:
-        check(lib().ginf_bp_plan_set_compact_data(self._h, 1 if enable else 0))
+    def set_compact_data(self, enable) -> None:
+        """True/1: the narrowest storage that fits (8-bit at levels 0-1, else
+        16-bit, else T); 2: at most 16-bit; False/0: always T."""
+        check(lib().ginf_bp_plan_set_compact_data(self._h, int(enable)))
 
     def set_pairs(self, enable) -> None:
         """Fused pairs of half-sweeps (ginf_bp_plan_set_pairs): True, False, or
@@ -223,10 +226,15 @@ class Plan:
         return bool(v.value)
 
     def compact_data(self) -> bool:
-        """Whether the last run stored the data costs as 16-bit integers."""
+        """Whether the last run stored the data costs as 16- or 8-bit integers."""
+        return self.data_form() != 0
+
+    def data_form(self) -> int:
+        """Storage of the last run's data costs: 0 T, 1 16-bit, 2 8-bit at
+        levels 0 and 1 (16-bit above)."""
         v = ctypes.c_int()
         check(lib().ginf_bp_plan_compact_data(self._h, ctypes.byref(v)))
-        return bool(v.value)
+        return v.value
 
     def set_sweep_timing(self, enable: bool) -> None:
         check(lib().ginf_bp_plan_set_sweep_timing(self._h, 1 if enable else 0))
@@ -237,6 +245,14 @@ class Plan:
         check(lib().ginf_bp_plan_sweep_stats(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(up),
                                              1 if reset else 0))
         return ms.value, n.value, up.value
+
+    def sweep_level_updates(self):
+        """Node-label updates of the timed half-sweeps per level (before a
+        resetting sweep_stats)."""
+        n = max(32, self.desc.num_levels)
+        buf = (ctypes.c_double * n)()
+        check(lib().ginf_bp_plan_sweep_level_updates(self._h, buf))
+        return [buf[t] for t in range(self.desc.num_levels)]
 
     def updates_per_run(self) -> float:
         u = ctypes.c_double()

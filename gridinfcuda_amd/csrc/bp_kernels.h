@@ -96,22 +96,25 @@ hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, cons
                            void *dst0, void *dst1, hipStream_t st);
 // data16: store the level data as 16-bit integers; a value without an exact
 // 16-bit form sets *flag (which the caller clears beforehand) and is stored as 0.
+// data8 (label-quad layout): the same with 8-bit integers, values in [0, 255].
 // only_if: when non-null the kernel does its work only if *only_if != 0 (the
-// redo in T after a 16-bit attempt that did not fit).
+// redo in a wider form after an attempt that did not fit).
 hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
                                const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
                                void *d11, hipStream_t st, bool data16 = false, int *flag = nullptr,
-                               const int *only_if = nullptr);
+                               const int *only_if = nullptr, bool data8 = false);
 // The label-quad relayout that also writes the costs min-pooled over 4 x 4
 // blocks, times `scale`, as BF16 NHWC [(H+3)/4][(W+3)/4][L] into mp (the
 // warm-start network's input); hipErrorNotSupported for other layouts.
 hipError_t launch_relayout_quad_mp(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
                                    const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1,
                                    void *d10, void *d11, hipStream_t st, bool data16, int *flag, uint16_t *mp,
-                                   float scale);
+                                   float scale, bool data8 = false);
+// src8 (label-quad layout, with data16): the fine level is stored in 8 bits,
+// the coarse one in 16.
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
                           const LevelGeom &cg, int C, int G, hipStream_t st, bool data16 = false, int *flag = nullptr,
-                          const int *only_if = nullptr);
+                          const int *only_if = nullptr, bool src8 = false);
 hipError_t launch_upsample(int dtype, const UpsampleArgs &u, hipStream_t st);
 // Batched persistent half-sweep (sweep_batch.hip) over instances of label
 // capacity lmax (16, 32, 64, 96 or 128), all with L == lmax when `full`:

@@ -43,23 +43,25 @@ __global__ __launch_bounds__(256) void k_relayout(const T *__restrict__ src, int
 #define GINF_RELAYOUT_TW64_MAX_LP 64
 #endif
 
-// Storage of a data cost: T itself, or a 16-bit unsigned integer when S is
-// uint16_t.  to_store records in *bad when a value has no exact 16-bit form
-// (not an integer in [0, 65535], or -0), and the plan then redoes the level
-// data in T.
-template <typename T>
-__device__ __forceinline__ bool fits_u16(T v) {
+// Storage of a data cost: T itself, or an unsigned integer when S is
+// uint16_t or uint8_t.  to_store records in *bad when a value has no exact
+// form in S (not an integer in [0, 65535] or [0, 255], or -0), and the plan
+// then redoes the level data in the next wider form.
+template <typename T, int MAXV>
+__device__ __forceinline__ bool fits_uint(T v) {
     if constexpr (std::is_same<T, float>::value)
-        return v >= T(0) && v <= T(65535) && v == __builtin_truncf(v) && !__builtin_signbit(v);
+        return v >= T(0) && v <= T(MAXV) && v == __builtin_truncf(v) && !__builtin_signbit(v);
     else
-        return v >= T(0) && v <= T(65535);
+        return v >= T(0) && v <= T(MAXV);
 }
 
 template <typename S, typename T>
 __device__ __forceinline__ S to_store(T v, bool &bad) {
-    if constexpr (sizeof(S) == 2) {
-        bad |= !fits_u16(v);
-        return (S)(uint32_t)(fits_u16(v) ? v : T(0));
+    if constexpr (sizeof(S) < sizeof(T)) {
+        constexpr int MAXV = sizeof(S) == 1 ? 255 : 65535;
+        const bool ok = fits_uint<T, MAXV>(v);
+        bad |= !ok;
+        return (S)(uint32_t)(ok ? v : T(0));
     } else {
         return v;
     }
@@ -67,7 +69,7 @@ __device__ __forceinline__ S to_store(T v, bool &bad) {
 
 template <typename S, typename T>
 __device__ __forceinline__ T from_store(S v) {
-    if constexpr (sizeof(S) == 2) return T((uint32_t)v);
+    if constexpr (sizeof(S) < sizeof(T)) return T((uint32_t)v);
     else return v;
 }
 
@@ -427,12 +429,14 @@ __global__ __launch_bounds__(256) void k_relayout_quad_mp(const T *__restrict__ 
 
 // Data-cost pyramid level for the label-quad layout: the rule of k_pyramid,
 // one thread per (plane i, coarse node) moving the node's label quad as one
-// vector per child (8 B in 16-bit storage, 16 B in T), grid-stride.
-template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_pyramid_quad(const S *f0, const S *f1, LevelGeom fg, S *c0, S *c1,
+// vector per child (4 B in 8-bit storage, 8 B in 16-bit, 16 B in T),
+// grid-stride.  The fine level is stored in SI, the coarse one in S.
+template <typename T, typename S, typename SI = S>
+__global__ __launch_bounds__(256) void k_pyramid_quad(const SI *f0, const SI *f1, LevelGeom fg, S *c0, S *c1,
                                                       LevelGeom cg, int C, int *flag, const int *only_if) {
     if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
     typedef S s4 __attribute__((ext_vector_type(4)));
+    typedef SI si4 __attribute__((ext_vector_type(4)));
     const int64_t nodes = (int64_t)cg.w * cg.h;
     const int64_t total = nodes * C;
     bool bad = false;
@@ -444,19 +448,19 @@ __global__ __launch_bounds__(256) void k_pyramid_quad(const S *f0, const S *f1, 
         const bool ex = 2 * X + 1 < fg.w, ey = 2 * Y + 1 < fg.h;
         const int64_t e00 = elem_index((int64_t)(2 * Y) * fg.whp + X, i, 0, C, 4);
         const int64_t e01 = elem_index((int64_t)(2 * Y + 1) * fg.whp + X, i, 0, C, 4);
-        const s4 v00 = *(const s4 *)(f0 + e00);
-        s4 v10 = s4(0), v01 = s4(0), v11 = s4(0);
-        if (ex) v10 = *(const s4 *)(f1 + e00);
-        if (ey) v01 = *(const s4 *)(f1 + e01);
-        if (ex && ey) v11 = *(const s4 *)(f0 + e01);
+        const si4 v00 = *(const si4 *)(f0 + e00);
+        si4 v10 = si4(0), v01 = si4(0), v11 = si4(0);
+        if (ex) v10 = *(const si4 *)(f1 + e00);
+        if (ey) v01 = *(const si4 *)(f1 + e01);
+        if (ex && ey) v11 = *(const si4 *)(f0 + e01);
         s4 o;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             T acc = T(0);
-            acc = acc + from_store<S, T>(v00[q]);
-            if (ex) acc = acc + from_store<S, T>(v10[q]);
-            if (ey) acc = acc + from_store<S, T>(v01[q]);
-            if (ex && ey) acc = acc + from_store<S, T>(v11[q]);
+            acc = acc + from_store<SI, T>(v00[q]);
+            if (ex) acc = acc + from_store<SI, T>(v10[q]);
+            if (ey) acc = acc + from_store<SI, T>(v01[q]);
+            if (ex && ey) acc = acc + from_store<SI, T>(v11[q]);
             o[q] = to_store<S>(acc, bad);
         }
         S *dst = ((X + Y) & 1) ? c1 : c0;
@@ -492,7 +496,7 @@ hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, cons
 hipError_t launch_relayout_quad_mp(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
                                    const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1,
                                    void *d10, void *d11, hipStream_t st, bool data16, int *flag, uint16_t *mp,
-                                   float scale) {
+                                   float scale, bool data8) {
     if (!(ly.interleave && ly.G == 4)) return hipErrorNotSupported;
     const dim3 grid((unsigned)((W + 511) / 512), (unsigned)((H + 3) / 4), (unsigned)ly.C), block(256);
     if (ly.C > 65535 || (H + 3) / 4 > 65535) return hipErrorInvalidValue;
@@ -500,7 +504,8 @@ hipError_t launch_relayout_quad_mp(int dtype, const void *src, int W, int H, int
 #define GINF_RQM(TT, SS)                                                                                         \
     hipLaunchKernelGGL((k_relayout_quad_mp<TT, SS>), grid, block, 0, st, (const TT *)src, W, H, L, ly.C, g0,      \
                        (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, mp, w4, scale)
-    if (data16) { if (dtype == 0) GINF_RQM(float, uint16_t); else GINF_RQM(int, uint16_t); }
+    if (data8) { if (dtype == 0) GINF_RQM(float, uint8_t); else GINF_RQM(int, uint8_t); }
+    else if (data16) { if (dtype == 0) GINF_RQM(float, uint16_t); else GINF_RQM(int, uint16_t); }
     else { if (dtype == 0) GINF_RQM(float, float); else GINF_RQM(int, int); }
 #undef GINF_RQM
     return hipGetLastError();
@@ -508,7 +513,7 @@ hipError_t launch_relayout_quad_mp(int dtype, const void *src, int W, int H, int
 
 hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
                                const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
-                               void *d11, hipStream_t st, bool data16, int *flag, const int *only_if) {
+                               void *d11, hipStream_t st, bool data16, int *flag, const int *only_if, bool data8) {
     if (ly.interleave && ly.G == 4) {  // label quads: register-only relayout
         const dim3 grid((unsigned)((W + 511) / 512), (unsigned)((H + 1) / 2), only_if ? 1u : (unsigned)ly.C),
             block(256);
@@ -516,7 +521,8 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
 #define GINF_RQ(TT, SS)                                                                                          \
         hipLaunchKernelGGL((k_relayout_quad<TT, SS>), grid, block, 0, st, (const TT *)src, W, H, L, ly.C, g0,     \
                            (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, only_if)
-        if (data16) { if (dtype == 0) GINF_RQ(float, uint16_t); else GINF_RQ(int, uint16_t); }
+        if (data8) { if (dtype == 0) GINF_RQ(float, uint8_t); else GINF_RQ(int, uint8_t); }
+        else if (data16) { if (dtype == 0) GINF_RQ(float, uint16_t); else GINF_RQ(int, uint16_t); }
         else { if (dtype == 0) GINF_RQ(float, float); else GINF_RQ(int, int); }
 #undef GINF_RQ
         return hipGetLastError();
@@ -527,9 +533,9 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
     if (lds > 160 * 1024 || (tw / 2) * ly.G > 256) {
         // Layouts past the LDS tile (32 or 64 lanes per node, or more than
         // about 600 labels): per-column relayout of level 0, then level 1
-        // summed from it.  Only T storage reaches here (16-bit data costs are
-        // a quad-layout form).
-        if (data16 || flag || only_if) return hipErrorInvalidValue;
+        // summed from it.  Only T storage reaches here (16- and 8-bit data
+        // costs are quad-layout forms).
+        if (data16 || data8 || flag || only_if) return hipErrorInvalidValue;
         const dim3 grid((unsigned)((W + 255) / 256), (unsigned)H), block(256);
         if (dtype == 0)
             hipLaunchKernelGGL(k_relayout<float>, grid, block, 0, st, (const float *)src, W, H, L, ly, g0,
@@ -542,6 +548,7 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
             e = launch_pyramid(dtype, d00, d01, g0, d10, d11, g1, ly.C, ly.G, st, false, nullptr, nullptr);
         return e;
     }
+    if (data8) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((W + tw - 1) / tw), (unsigned)((H + 1) / 2)), block(256);
 #define GINF_RP(TT, SS, TWW)                                                                                     \
     hipLaunchKernelGGL((k_relayout_pyr<TT, SS, TWW>), grid, block, lds, st, (const TT *)src, W, H, L, ly, g0,      \
@@ -559,7 +566,8 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
 
 hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const LevelGeom &fg, void *c0, void *c1,
                           const LevelGeom &cg, int C, int G, hipStream_t st, bool data16, int *flag,
-                          const int *only_if) {
+                          const int *only_if, bool src8) {
+    if (src8 && !(data16 && G == 4)) return hipErrorInvalidValue;
     if (G == 4) {
         // four lanes per node: one vector per child (the sums are per element,
         // so the label order of the layout does not matter)
@@ -571,7 +579,14 @@ hipError_t launch_pyramid(int dtype, const void *f0, const void *f1, const Level
 #define GINF_PQ(TT, SS)                                                                                           \
         hipLaunchKernelGGL((k_pyramid_quad<TT, SS>), grid, block, 0, st, (const SS *)f0, (const SS *)f1, fg,       \
                            (SS *)c0, (SS *)c1, cg, C, flag, only_if)
-        if (data16) { if (dtype == 0) GINF_PQ(float, uint16_t); else GINF_PQ(int, uint16_t); }
+        if (src8) {  // 8-bit fine level (level 1 of an 8-bit run), 16-bit coarse level
+            if (dtype == 0)
+                hipLaunchKernelGGL((k_pyramid_quad<float, uint16_t, uint8_t>), grid, block, 0, st, (const uint8_t *)f0,
+                                   (const uint8_t *)f1, fg, (uint16_t *)c0, (uint16_t *)c1, cg, C, flag, only_if);
+            else
+                hipLaunchKernelGGL((k_pyramid_quad<int, uint16_t, uint8_t>), grid, block, 0, st, (const uint8_t *)f0,
+                                   (const uint8_t *)f1, fg, (uint16_t *)c0, (uint16_t *)c1, cg, C, flag, only_if);
+        } else if (data16) { if (dtype == 0) GINF_PQ(float, uint16_t); else GINF_PQ(int, uint16_t); }
         else { if (dtype == 0) GINF_PQ(float, float); else GINF_PQ(int, int); }
 #undef GINF_PQ
         return hipGetLastError();

@@ -89,6 +89,9 @@ struct SweepArgs {
     uint32_t data_plane_bytes;  // bytes between data-cost planes in 16-bit storage (half of plane_bytes;
                                 // 16-bit element offsets are the T offsets halved)
     const int *fit_flag;     // 16-bit data path: 0 = the level data is stored in 16 bits, else in T
+    const int *fit8_flag;    // levels 0 and 1 of a plan whose run tried 8-bit storage: with *fit_flag
+                             // 0, *fit8_flag 0 = this level's data is stored in 8 bits (element
+                             // offsets a quarter of T's), else in 16; null = never 8 bits
     // Prediction-sourced first half-sweep of a seeded level (quad kernel, FP32):
     // the incoming message from neighbour n in direction od is
     // pred[min(ny >> pred_shift, pred_h - 1)][min(nx >> pred_shift, pred_w - 1)][od * L + l]

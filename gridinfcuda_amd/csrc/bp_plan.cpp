@@ -140,11 +140,17 @@ struct ginf_bp_plan_s {
     int device = 0;
     // 16-bit data costs (quad kernel): on by default; a run keeps them when
     // the relayout and pyramid find every cost of every level to be an integer
-    // in [0, 65535] (flag_dev stays 0), else it redoes the level data in T.
+    // in [0, 65535] (flag_dev[0] stays 0), else it redoes the level data in T.
+    // 8-bit data costs at levels 0 and 1 (compact8, on by default): the run
+    // first stores those two levels in bytes (16 bits above) and keeps them
+    // when every cost there is an integer in [0, 255] (flag_dev[1] stays 0),
+    // else it redoes all levels in 16 bits.
     bool compact = true;
+    bool compact8 = true;
     bool compact_last = false;
+    bool compact8_last = false;
     int prepared_start = -1;  // highest level whose data the last plan_prepare built
-    int *flag_dev = nullptr;
+    int *flag_dev = nullptr;  // [0]: a cost without a 16-bit form; [1]: one at levels 0-1 without an 8-bit form
     // fused pairs of half-sweeps (ginf_bp_plan_set_pairs) and, per level, the
     // second set of colour-0 outbox arrays they alternate with
     int pair_mode = -1;  // -1: fused where measured faster (label capacity <= 32), 0: off, 1: on
@@ -152,7 +158,7 @@ struct ginf_bp_plan_s {
     std::vector<void *> alt;
     // sweep timing (profiling hook)
     bool timing = false;
-    struct Timed { hipEvent_t beg, end; double updates; };
+    struct Timed { hipEvent_t beg, end; double updates; int level; };
     std::vector<Timed> timed;
     std::vector<hipEvent_t> spare;
 };
@@ -288,13 +294,14 @@ int64_t interior_of_color(int w, int h, int color) {
     return n;
 }
 
-// Whether the last run kept its data costs in 16-bit storage (waits for the device).
+// The storage form of the last run's data costs (waits for the device): 0 T,
+// 1 16-bit, 2 8-bit at levels 0 and 1 and 16-bit above.
 hipError_t int16_used(ginf_bp_plan p, int *used) {
     *used = 0;
     if (!p->compact_last || !p->flag_dev) return hipSuccess;
-    int f = 1;
-    hipError_t e = hipMemcpy(&f, p->flag_dev, sizeof(int), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) *used = f == 0 ? 1 : 0;
+    int f[2] = {1, 1};
+    hipError_t e = hipMemcpy(f, p->flag_dev, sizeof f, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) *used = f[0] != 0 ? 0 : (p->compact8_last && f[1] == 0) ? 2 : 1;
     return e;
 }
 
@@ -365,31 +372,42 @@ int plan_prepare(ginf_bp_plan p, const void *data_dev, hipStream_t st, int start
     const bool two = start >= 1;  // a run that starts at level 0 needs no level-1 sums
     const bool with_mp = mp && ly.interleave && ly.G == 4;
     if (mp_done) *mp_done = with_mp;
-    auto prepare = [&](bool d16, int *flag, const int *only_if) -> int {
+    // bits: 8 (levels 0 and 1 in bytes, misfits to flag8; levels 2+ in 16
+    // bits, misfits to flag), 16 or 32 (T)
+    auto prepare = [&](int bits, int *flag, int *flag8, const int *only_if) -> int {
+        const bool d16 = bits <= 16, d8 = bits == 8;
+        int *f01 = d8 ? flag8 : flag;
         if (with_mp && !only_if)
             GINF_HIP_TRY(launch_relayout_quad_mp(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0],
                                                  p->lv[0].data[1], two, p->geom[two ? 1 : 0],
                                                  p->lv[two ? 1 : 0].data[0], p->lv[two ? 1 : 0].data[1], st, d16,
-                                                 flag, mp, mp_scale));
+                                                 f01, mp, mp_scale, d8));
         else
             GINF_HIP_TRY(launch_relayout_pyr(dt, data_dev, W, H, L, ly, p->geom[0], p->lv[0].data[0],
                                              p->lv[0].data[1], two, p->geom[two ? 1 : 0], p->lv[two ? 1 : 0].data[0],
-                                             p->lv[two ? 1 : 0].data[1], st, d16, flag, only_if));
+                                             p->lv[two ? 1 : 0].data[1], st, d16, f01, only_if, d8));
         for (int t = 2; t <= start; ++t)
             GINF_HIP_TRY(launch_pyramid(dt, p->lv[t - 1].data[0], p->lv[t - 1].data[1], p->geom[t - 1],
                                         p->lv[t].data[0], p->lv[t].data[1], p->geom[t], C, G, st, d16, flag,
-                                        only_if));
+                                        only_if, d8 && t == 2));
         return GINF_OK;
     };
     const bool d16 = p->compact && p->shape.quad && p->flag_dev;
+    const bool d8 = d16 && p->compact8;
     if (d16) {
-        GINF_HIP_TRY(launch_fill_u32((uint32_t *)p->flag_dev, 1, 0u, st));
-        if (int rc = prepare(true, p->flag_dev, nullptr)) return rc;
-        if (int rc = prepare(false, nullptr, p->flag_dev)) return rc;
-    } else if (int rc = prepare(false, nullptr, nullptr)) {
+        GINF_HIP_TRY(launch_fill_u32((uint32_t *)p->flag_dev, 2, 0u, st));
+        if (d8) {
+            if (int rc = prepare(8, p->flag_dev, p->flag_dev + 1, nullptr)) return rc;
+            if (int rc = prepare(16, p->flag_dev, nullptr, p->flag_dev + 1)) return rc;  // levels 0-1 did not fit
+        } else if (int rc = prepare(16, p->flag_dev, nullptr, nullptr)) {
+            return rc;
+        }
+        if (int rc = prepare(32, nullptr, nullptr, p->flag_dev)) return rc;
+    } else if (int rc = prepare(32, nullptr, nullptr, nullptr)) {
         return rc;
     }
     p->compact_last = d16;
+    p->compact8_last = d8;
     p->prepared_start = start;
     return GINF_OK;
 }
@@ -469,6 +487,7 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
             a.tile_bytes = (uint32_t)(a.tile * 4);
             a.data_plane_bytes = a.plane_bytes / 2;
             a.fit_flag = d16 ? p->flag_dev : nullptr;
+            a.fit8_flag = (d16 && p->compact8_last && t <= 1) ? p->flag_dev + 1 : nullptr;
             a.out_base = p->lv[t].msgs[color][0];
             if (src_level == t) {
                 a.in_base = p->lv[t].msgs[1 - color][0];
@@ -522,7 +541,7 @@ int plan_run_ex(ginf_bp_plan p, const void *data_dev, int32_t *labels_dev, hipSt
             const bool labels_i = (t == 0) && (i >= N - 2);
             const bool labels_j = pair && (t == 0) && (i + 1 >= N - 2);
             const SweepArgs a = make_args(i);
-            ginf_bp_plan_s::Timed tm{nullptr, nullptr, 0.0};
+            ginf_bp_plan_s::Timed tm{nullptr, nullptr, 0.0, t};
             // Only the steady-state kernel instance (no zero-input, no labels) is
             // timed, so the figure matches a per-kernel profiler summary.
             if (p->timing && !zeroin && !labels_i && !labels_j && !(from_pred && i == 0)) {
@@ -683,7 +702,7 @@ int ginf_bp_plan_create(const ginf_bp_desc *desc, const void *sm_table_host, gin
         e = hipMemcpy(p->table_dev, sm_table_host, (size_t)L * L * es, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess && p->shape.quad) {
-        e = hipMalloc((void **)&p->flag_dev, sizeof(int));
+        e = hipMalloc((void **)&p->flag_dev, 2 * sizeof(int));
     }
     if (e != hipSuccess) {
         if (p->flag_dev) hipFree(p->flag_dev);
@@ -718,7 +737,9 @@ int ginf_bp_plan_workspace_bytes(ginf_bp_plan plan, size_t *bytes) {
 
 int ginf_bp_plan_set_compact_data(ginf_bp_plan plan, int enable) {
     if (!plan) return set_error(GINF_ERR_INVALID, "NULL plan");
+    if (enable < 0 || enable > 2) return set_error(GINF_ERR_INVALID, "compact data mode %d (0, 1 or 2)", enable);
     plan->compact = enable != 0;
+    plan->compact8 = enable == 1;
     return GINF_OK;
 }
 
@@ -764,6 +785,13 @@ int ginf_bp_plan_sweep_stats(ginf_bp_plan p, double *total_ms, int64_t *launches
         for (auto &t : p->timed) { p->spare.push_back(t.beg); p->spare.push_back(t.end); }
         p->timed.clear();
     }
+    return GINF_OK;
+}
+
+int ginf_bp_plan_sweep_level_updates(ginf_bp_plan p, double *updates_by_level) {
+    if (!p || !updates_by_level) return set_error(GINF_ERR_INVALID, "NULL argument");
+    for (int t = 0; t < p->levels; ++t) updates_by_level[t] = 0.0;
+    for (auto &t : p->timed) updates_by_level[t.level] += t.updates;
     return GINF_OK;
 }
 
@@ -911,19 +939,23 @@ int ginf_bp_plan_dump_data(ginf_bp_plan p, int level, void *data_host) {
     const size_t n = (size_t)C * g.nplane * G;
     std::vector<uint32_t> buf(n);
     GINF_HIP_TRY(hipDeviceSynchronize());
-    int used16 = 0;
-    GINF_HIP_TRY(int16_used(p, &used16));
-    std::vector<uint16_t> buf16(used16 ? n : 0);
+    int form = 0;
+    GINF_HIP_TRY(int16_used(p, &form));
+    const int bytes = form == 0 ? 4 : (form == 2 && level <= 1) ? 1 : 2;
+    std::vector<uint16_t> buf16(bytes == 2 ? n : 0);
+    std::vector<uint8_t> buf8(bytes == 1 ? n : 0);
     uint32_t *dst = (uint32_t *)data_host;
     for (int c = 0; c < 2; ++c) {
-        if (used16) {  // 16-bit storage of the last run: widen to T
-            GINF_HIP_TRY(hipMemcpy(buf16.data(), p->lv[level].data[c], n * 2, hipMemcpyDeviceToHost));
+        if (bytes < 4) {  // 16- or 8-bit storage of the last run: widen to T
+            if (bytes == 2) GINF_HIP_TRY(hipMemcpy(buf16.data(), p->lv[level].data[c], n * 2, hipMemcpyDeviceToHost));
+            else GINF_HIP_TRY(hipMemcpy(buf8.data(), p->lv[level].data[c], n, hipMemcpyDeviceToHost));
             for (size_t k = 0; k < n; ++k) {
+                const uint32_t v = bytes == 2 ? buf16[k] : buf8[k];
                 if (p->desc.dtype == GINF_F32) {
-                    const float f = (float)buf16[k];
+                    const float f = (float)v;
                     std::memcpy(&buf[k], &f, 4);
                 } else {
-                    buf[k] = buf16[k];
+                    buf[k] = v;
                 }
             }
         } else {

@@ -42,6 +42,9 @@ __device__ __forceinline__ int quad_bcast(int v) {
 //         of every level is an integer in [0, 65535]) and a lane loads its 4
 //         labels' costs as 8 bytes; otherwise they were stored in T.  The flag
 //         is uniform, so the branch never diverges.
+//   D8    (with D16) also the 8-bit form: where a.fit8_flag is set and reads
+//         0 as well, this level's costs are integers in [0, 255] stored in
+//         bytes, and a lane loads its 4 labels' costs as 4 bytes.
 // Layout: label quads in slot tiles, element (l, slot) at
 // elem_index(slot, l/4, l%4, C, 4): one lane moves 4 labels (16 B) per
 // access, 16 consecutive nodes of a wave cover 256 contiguous bytes of every
@@ -59,7 +62,7 @@ __device__ __forceinline__ int quad_bcast(int v) {
 //         outboxes ginf_bp_plan_seed would have stored, read where they come
 //         from (FP32 only; a lane's label quad is 16 contiguous bytes of the
 //         prediction row, L % 4 == 0).
-template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool use16, bool PRED = false>
+template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool use16, bool PRED = false, bool use8 = false>
 __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slot, const int d) {
     typedef T v4 __attribute__((ext_vector_type(4)));
     constexpr int LQ = LMAX / 4;
@@ -75,8 +78,9 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
     const __amdgpu_buffer_rsrc_t irs = __builtin_amdgcn_make_buffer_rsrc((void *)a.in_base, 0, a.in_span, 0x00020000);
     const __amdgpu_buffer_rsrc_t ors = __builtin_amdgcn_make_buffer_rsrc(a.out_base, 0, 4u * a.arr_bytes, 0x00020000);
     const uint32_t self_off = (uint32_t)(slot >> kTileShift) * a.tile_bytes + (uint32_t)(slot & (kTileSlots - 1)) * 16u;
-    const uint32_t data_off = use16 ? self_off / 2u : self_off;  // 16-bit storage: every offset halves
-    const uint32_t data_plane = use16 ? a.data_plane_bytes : a.plane_bytes;
+    // 16-bit storage: every offset halves; 8-bit: a quarter
+    const uint32_t data_off = use8 ? self_off / 4u : use16 ? self_off / 2u : self_off;
+    const uint32_t data_plane = use8 ? a.data_plane_bytes / 2u : use16 ? a.data_plane_bytes : a.plane_bytes;
     uint32_t in_off = 0;
     bool in_live = true;
     if constexpr (PRED) {
@@ -115,7 +119,13 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
                 for (int r = 0; r < 4; ++r) m4[r] = in_live ? m4[r] * a.pred_inv_scale : T(0);
             }
             v4 d4;
-            if constexpr (use16) {
+            if constexpr (use8) {
+                const uint32_t h = __builtin_amdgcn_raw_buffer_load_b32(drs, data_off + (uint32_t)j * data_plane, 0, GINF_QUAD_LOAD_POLICY);
+                d4[0] = T(h & 0xFFu);
+                d4[1] = T((h >> 8) & 0xFFu);
+                d4[2] = T((h >> 16) & 0xFFu);
+                d4[3] = T(h >> 24);
+            } else if constexpr (use16) {
                 const auto h = __builtin_amdgcn_raw_buffer_load_b64(drs, data_off + (uint32_t)j * data_plane, 0, GINF_QUAD_LOAD_POLICY);
                 d4[0] = T(h[0] & 0xFFFFu);
                 d4[1] = T(h[0] >> 16);
@@ -195,10 +205,17 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
 // The two storage forms are separate straight-line bodies; the uniform flag
 // picks one per wave (a branch inside the unrolled label loop would stop the
 // compiler from issuing the loads ahead).
-template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool D16 = false, bool PRED = false>
+template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool D16 = false, bool PRED = false,
+          bool D8 = false>
 __device__ __forceinline__ void quad_node(const SweepArgs &a, const int slot, const int d) {
     if constexpr (D16) {
         if (__builtin_amdgcn_readfirstlane(*a.fit_flag) == 0) {
+            if constexpr (D8) {
+                if (a.fit8_flag && __builtin_amdgcn_readfirstlane(*a.fit8_flag) == 0) {
+                    quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, true, PRED, true>(a, slot, d);
+                    return;
+                }
+            }
             quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, true, PRED>(a, slot, d);
             return;
         }

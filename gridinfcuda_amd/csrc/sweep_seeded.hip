@@ -13,7 +13,7 @@ namespace {
 template <int LMAX, bool LABELS, bool FULL, bool D16>
 __global__ __launch_bounds__(256) void k_sweep_seeded(const SweepArgs a) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    quad_node<float, LMAX, false, LABELS, FULL, D16, true>(a, gid >> 2, gid & 3);
+    quad_node<float, LMAX, false, LABELS, FULL, D16, true, D16>(a, gid >> 2, gid & 3);
 }
 
 template <int LMAX>

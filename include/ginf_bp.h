@@ -83,14 +83,17 @@ int ginf_bp_plan_run(ginf_bp_plan plan, const void *data_dev, int32_t *labels_de
 
 int ginf_bp_plan_destroy(ginf_bp_plan plan);
 
-/* 16-bit data costs (no reference counterpart).  With the quad kernel a
- * plan stores the pyramid's data costs as 16-bit integers whenever every
- * cost of every level is an integer in [0, 65535], and as T otherwise; the
- * check and the choice happen on the device during the run (the sweeps read
- * the outcome), both forms give identical results, and the 16-bit form moves
- * 34 instead of 36 bytes per node-label update.  Enabled by default; enable
- * = 0 always stores T.  ginf_bp_plan_compact_data reports whether the last
- * run kept 16-bit costs (it waits for the device). */
+/* 16- and 8-bit data costs (no reference counterpart).  With the quad
+ * kernel a plan stores the pyramid's data costs as 16-bit integers whenever
+ * every cost of every level is an integer in [0, 65535], and as T otherwise;
+ * levels 0 and 1 are stored in 8 bits when every cost of both is an integer
+ * in [0, 255].  The checks and the choice happen on the device during the
+ * run (the sweeps read the outcome), every form gives identical results, and
+ * a node-label update moves 36 bytes with T costs, 34 with 16-bit and 33
+ * with 8-bit costs.  enable: 1 (default) the narrowest form that fits, 2 at
+ * most 16-bit storage, 0 always T.  ginf_bp_plan_compact_data reports the
+ * last run's form (it waits for the device): 0 T, 1 16-bit, 2 8-bit at
+ * levels 0 and 1 and 16-bit above. */
 int ginf_bp_plan_set_compact_data(ginf_bp_plan plan, int enable);
 int ginf_bp_plan_compact_data(ginf_bp_plan plan, int *last_run_compact);
 
@@ -156,6 +159,11 @@ int ginf_bp_plan_dump_msgs(ginf_bp_plan plan, int level, void *msgs_host);
 int ginf_bp_plan_set_sweep_timing(ginf_bp_plan plan, int enable);
 int ginf_bp_plan_sweep_stats(ginf_bp_plan plan, double *total_ms, int64_t *launches, double *node_label_updates,
                              int reset);
+/* The timed launches' node-label updates split by level (updates_by_level
+ * holds one entry per pyramid level; call before a resetting
+ * ginf_bp_plan_sweep_stats), so that a caller can weigh each level's
+ * updates by the bytes its data-cost form moves (no reference counterpart). */
+int ginf_bp_plan_sweep_level_updates(ginf_bp_plan plan, double *updates_by_level);
 
 /* Node-label updates one ginf_bp_plan_run performs: the sum over levels and
  * half-sweeps of (interior nodes of the moving colour) x L. */

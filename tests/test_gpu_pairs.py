@@ -44,8 +44,9 @@ def test_pairs_bit_exact(oracle_mod, H, W, L, levels, iters, kind, dt, norm):
     data = rng.integers(0, 40, size=(L, H, W)).astype(dtype)
     table = _table(oracle_mod, kind, L, dtype)
     init = rng.integers(0, L, size=(H, W)).astype(np.int32)
+    # the data-cost storage forms in turn: narrowest that fits (8-bit here), at most 16-bit, T
     lab, msgs, plan = run_plan(data, table, SM[kind], levels, iters, norm=norm, labels_init=init, dump_level=0,
-                               kernel=2, compact=bool((H + W + L) & 1), pairs=True)
+                               kernel=2, compact=[1, 2, 0][(H + W + L) % 3], pairs=True)
     assert plan.pairs() == (L <= 128)
     ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, SM[kind], levels, iters, norm=norm, labels_init=init,
                                           return_state=True)

@@ -38,7 +38,7 @@ def stereo(seed, W, H, L, rects, dmax, trunc, dtype):
 
 @pytest.mark.parametrize("dtype,norm", [(np.int32, 0), (np.float32, 1), (np.float32, 0), (np.int32, 1)])
 @pytest.mark.parametrize("lanes,kernel,compact", [(0, 0, None), (1, 1, None), (2, 1, None), (4, 1, None), (0, 2, True),
-                                                  (0, 2, False)])
+                                                  (0, 2, 2), (0, 2, False)])
 def test_config1_bit_exact(oracle_mod, dtype, norm, lanes, kernel, compact):
     """128x96, 16 labels, TL (10,20), 10 half-sweeps, stereo seed 1."""
     say(f"config1 dtype={np.dtype(dtype).name} norm={norm} lanes={lanes} kernel={kernel} compact={compact}")
@@ -113,8 +113,9 @@ def test_linear_bit_exact(oracle_mod, H, W, L, lanes, levels, iters, kind, dt, n
     data = rng.integers(0, 40, size=(L, H, W)).astype(dtype)
     table = table_for(oracle_mod, kind, L, dtype)
     init = rng.integers(0, L, size=(H, W)).astype(np.int32)
-    # quad cases alternate between 16-bit and T storage of the data costs
-    compact = bool((H + W + L) & 1) if kernel == 2 else None
+    # quad cases cycle through the storage forms of the data costs: the
+    # narrowest that fits (8-bit here), at most 16-bit, and T
+    compact = [1, 2, 0][(H + W + L) % 3] if kernel == 2 else None
     if expect_unsupported(L, lanes, kernel):
         with pytest.raises(G.GinfError) as e:
             run_plan(data, table, SM[kind], levels, iters, norm=norm, lanes=lanes, labels_init=init, dump_level=0,
@@ -216,9 +217,10 @@ def test_hbp_config2_full_size(oracle_mod):
     data = stereo(2, 1920, 1080, 128, 64, 127, 40, np.float32)
     table = oracle_mod.table_trunc_linear(128, 8, 32)
     ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, 2, 5, 10, norm=0, threads=16, return_state=True)
-    for compact in (True, False):  # 16-bit and 32-bit storage of the data costs
+    # 8-bit (levels 0-1; the costs are at most 40 there), 16-bit and 32-bit storage of the data costs
+    for compact, form in ((1, 2), (2, 1), (0, 0)):
         lab, msgs, plan = run_plan(data, table, 2, 5, 10, norm=0, dump_level=0, compact=compact)
-        assert plan.compact_data() == compact
+        assert plan.data_form() == form
         np.testing.assert_array_equal(lab, ref_lab)
         assert np.array_equal(msgs, ref_msgs), f"compact={compact}: {int((msgs != ref_msgs).sum())} messages differ"
         del msgs, plan
@@ -314,16 +316,33 @@ def _pyramid_ref(oracle_mod, data, levels):
     return out
 
 
-@pytest.mark.parametrize("case", ["fits", "fraction", "coarse_overflow", "negative", "int_overflow"])
+# storage form each case must end in: 2 = 8-bit at levels 0-1 and 16-bit
+# above, 1 = 16-bit, 0 = T
+COMPACT_CASES = {"fits": 2, "coarse16": 2, "fine16": 1, "coarse8": 1, "cap16": 1, "fraction": 0,
+                 "coarse_overflow": 0, "negative": 0, "int_overflow": 0}
+
+
+@pytest.mark.parametrize("case", list(COMPACT_CASES))
 def test_compact_data_fallback_exact(oracle_mod, case):
-    """16-bit storage is used exactly when every level's costs are integers in
-    [0, 65535]; otherwise the plan falls back to T storage.  Either way the
-    pyramid (dump_data), messages and labels equal the oracle's."""
+    """Levels 0 and 1 are stored in 8 bits exactly when their costs are
+    integers in [0, 255] (level 2 then summed from the 8-bit level 1), every
+    level in 16 bits when all costs are integers in [0, 65535], else in T.
+    In every form the pyramid (dump_data), messages and labels equal the
+    oracle's."""
     rng = np.random.default_rng(17)
     L, H, W, levels, iters = 16, 37, 52, 3, 6
     dtype = np.int32 if case == "int_overflow" else np.float32
     data = rng.integers(0, 40, size=(L, H, W)).astype(dtype)
-    if case == "fraction":
+    compact = None
+    if case == "coarse16":  # levels 0-1 at most 252, level 2 up to 1008
+        data = rng.integers(56, 64, size=(L, H, W)).astype(dtype)
+    elif case == "fine16":  # one level-0 cost past 255
+        data[5, 9, 9] = 300
+    elif case == "coarse8":  # level 0 fits 8 bits, level-1 sums do not
+        data[:, :4, :4] = 200
+    elif case == "cap16":  # 8-bit storage switched off
+        compact = 2
+    elif case == "fraction":
         data[3, 10, 10] += 0.5
     elif case == "coarse_overflow":  # fits at level 0, sums exceed 65535 at level 1
         data[:, :4, :4] = 30000
@@ -334,8 +353,8 @@ def test_compact_data_fallback_exact(oracle_mod, case):
     table = oracle_mod.table_trunc_linear(L, 4, 12, dtype)
     say(f"compact data case {case}")
     ref_lab, ref_msgs, _ = oracle_mod.hbp(data, table, 2, levels, iters, norm=0, return_state=True)
-    lab, msgs, plan = run_plan(data, table, 2, levels, iters, norm=0, dump_level=0, kernel=2)
-    assert plan.compact_data() == (case == "fits")
+    lab, msgs, plan = run_plan(data, table, 2, levels, iters, norm=0, dump_level=0, kernel=2, compact=compact)
+    assert plan.data_form() == COMPACT_CASES[case]
     np.testing.assert_array_equal(lab, ref_lab)
     np.testing.assert_array_equal(msgs, ref_msgs)
     for t, ref in enumerate(_pyramid_ref(oracle_mod, data, levels)):
