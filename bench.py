@@ -191,8 +191,8 @@ def bench_batched(args) -> dict:
                "labels_equal_gpu": same, "seconds": round(secs, 3), "cpu_model": cpu_model()}
     value = U / (ms / 1000.0) / 1e9
     # the generator's costs are integers in [0, 40]: every instance keeps its
-    # data costs in 16-bit storage (34 B per update)
-    bpu = BYTES_PER_UPDATE - 2
+    # data costs in 8-bit storage (33 B per update)
+    bpu = BYTES_PER_UPDATE - 3
     achieved = bpu * U / (ms / 1000.0) / 1e9
     return {
         "metric": "Gnode-label msg updates/s", "value": round(value, 3), "unit": "Gnode-label msg updates/s",
@@ -205,8 +205,8 @@ def bench_batched(args) -> dict:
                    "tiles": {"slots_per_item": list(bt.tiles[0]), "blocks_per_cu": bt.tiles[1], "source": tuned}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "whole graph (k_sweep_batch16 + relayouts), 34 B per node-label update "
-                               "(16-bit data costs); traffic per replay"},
+                     "kernel": "whole graph (k_sweep_batch16 + relayouts), 33 B per node-label update "
+                               "(8-bit data costs); traffic per replay"},
         "cpu_baseline": cpu,
     }
 

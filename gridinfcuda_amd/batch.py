@@ -66,8 +66,10 @@ class Batch:
         lp = (ctypes.c_void_p * self.n)(*label_ptrs)
         G.check(G.lib().ginf_batch_run(self._h, dp, lp, ctypes.c_void_p(stream)))
 
-    def set_compact_data(self, enable: bool) -> None:
-        G.check(G.lib().ginf_batch_set_compact_data(self._h, 1 if enable else 0))
+    def set_compact_data(self, enable) -> None:
+        """True/1: the narrowest data-cost storage that fits (8-bit, else
+        16-bit, else T) per instance; 2: at most 16-bit; False/0: T."""
+        G.check(G.lib().ginf_batch_set_compact_data(self._h, int(enable)))
 
     def set_seeded(self, seeded: bool) -> None:
         G.check(G.lib().ginf_batch_set_seeded(self._h, 1 if seeded else 0))

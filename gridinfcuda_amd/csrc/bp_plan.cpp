@@ -1141,7 +1141,8 @@ struct ginf_batch_s {
     bool predicted = false;
     std::vector<ginf::SweepArgs> pred_args;
     ginf::SweepArgs *pred_args_dev = nullptr;
-    bool compact = true;  // 16-bit data costs per instance where they fit (each plan's fit flag)
+    bool compact = true;  // 16-bit data costs per instance where they fit (each plan's fit flags)
+    bool compact8 = true;  // and 8-bit ones where those fit
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -1258,17 +1259,20 @@ int batch_capture(ginf_batch b, const void *const *data_dev, int32_t *const *lab
         const ginf_bp_plan p = b->plans[i];
         hipStream_t gs = b->group_stream[b->inst_group[i]];
         const StateLayout ly = state_layout(p->shape, p->desc.num_labels);
-        auto relayout = [&](bool d16, int *flag, const int *only_if) {
+        auto relayout = [&](int bits, int *flag, const int *only_if) {
             return launch_relayout_pyr(b->dtype, data_dev[i], p->desc.width, p->desc.height, p->desc.num_labels, ly,
                                        p->geom[0], p->lv[0].data[0], p->lv[0].data[1], false, p->geom[0],
-                                       p->lv[0].data[0], p->lv[0].data[1], gs, d16, flag, only_if);
+                                       p->lv[0].data[0], p->lv[0].data[1], gs, bits <= 16, flag, only_if, bits == 8);
         };
         if (b->compact) {
-            // 16-bit costs where the instance's costs fit, else redone in T (decided on the device)
+            // 8-bit costs where the instance's costs fit, else redone in 16 bits, else in T (decided
+            // on the device; flag_dev[1] set = not 8-bit, flag_dev[0] set = not 16-bit either)
             if ((e = launch_fill_u32(p->flag_dev, 1, 0u, gs)) != hipSuccess) return fail(e);
-            if ((e = relayout(true, p->flag_dev, nullptr)) != hipSuccess) return fail(e);
-            if ((e = relayout(false, nullptr, p->flag_dev)) != hipSuccess) return fail(e);
-        } else if ((e = relayout(false, nullptr, nullptr)) != hipSuccess) {
+            if ((e = launch_fill_u32(p->flag_dev + 1, 1, b->compact8 ? 0u : 1u, gs)) != hipSuccess) return fail(e);
+            if (b->compact8 && (e = relayout(8, p->flag_dev + 1, nullptr)) != hipSuccess) return fail(e);
+            if ((e = relayout(16, p->flag_dev, b->compact8 ? p->flag_dev + 1 : nullptr)) != hipSuccess) return fail(e);
+            if ((e = relayout(32, nullptr, p->flag_dev)) != hipSuccess) return fail(e);
+        } else if ((e = relayout(32, nullptr, nullptr)) != hipSuccess) {
             return fail(e);
         }
         if (N <= 1) {
@@ -1366,6 +1370,7 @@ int ginf_batch_create(int n, const ginf_bp_desc *descs, const void *const *table
             a.tile_bytes = (uint32_t)(a.tile * 4);
             a.data_plane_bytes = a.plane_bytes / 2;
             a.fit_flag = p->flag_dev;
+            a.fit8_flag = p->flag_dev + 1;
             a.out_base = p->lv[0].msgs[c][0];
             a.in_base = p->lv[0].msgs[1 - c][0];
             a.in_span = (uint32_t)(4 * p->arr_bytes[0]);
@@ -1418,8 +1423,10 @@ int ginf_batch_run(ginf_batch b, const void *const *data_dev, int32_t *const *la
 
 int ginf_batch_set_compact_data(ginf_batch b, int enable) {
     if (!b) return set_error(GINF_ERR_INVALID, "NULL batch");
-    if (b->compact != (enable != 0)) batch_drop_graph(b);
+    if (enable < 0 || enable > 2) return set_error(GINF_ERR_INVALID, "compact data mode %d (0, 1 or 2)", enable);
+    if (b->compact != (enable != 0) || b->compact8 != (enable == 1)) batch_drop_graph(b);
     b->compact = enable != 0;
+    b->compact8 = enable == 1;
     return GINF_OK;
 }
 

@@ -288,8 +288,10 @@ int ginf_batch_updates_per_run(ginf_batch batch, double *node_label_updates);
  * stored by ginf_batch_seed (ginf_bp_plan_seed on the instance) instead of
  * zeros. */
 int ginf_batch_set_seeded(ginf_batch batch, int seeded);
-/* 16-bit data costs per instance where they fit, as ginf_bp_plan_set_compact_data
- * (default on; decided on the device for each instance during the run). */
+/* 16- or 8-bit data costs per instance where they fit, as
+ * ginf_bp_plan_set_compact_data (enable 1, the default: the narrowest form
+ * that fits; 2: at most 16 bits; 0: T; decided on the device for each
+ * instance during the run). */
 int ginf_batch_set_compact_data(ginf_batch batch, int enable);
 int ginf_batch_seed(ginf_batch batch, int instance, const float *pred_dev, int pw, int ph, int shift, float scale,
                     void *stream);

@@ -193,12 +193,13 @@ def test_batch_predictions_refused():
         with pytest.raises(G.GinfError):
             bt.set_predictions([(p.data_ptr(), 2, 2, 1.0) for _ in xs])
 
-@pytest.mark.parametrize("compact", [True, False])
+@pytest.mark.parametrize("compact", [1, 2, 0])
 def test_batch_data_storage_forms(oracle_mod, compact):
-    """Each instance keeps 16-bit data costs when they fit and falls back on
-    its own when they do not (here: one fractional and one out-of-range
-    instance among integer ones); with either setting every instance equals
-    the oracle."""
+    """Each instance keeps 8-bit data costs when they fit, else 16-bit ones,
+    and falls back on its own when they do not (here: one instance with a
+    cost past 255, one fractional and one out-of-range instance among ones
+    in [0, 40]); with every setting (narrowest form, at most 16 bits, T)
+    every instance equals the oracle."""
     torch = torch_cuda()
     say(f"batch data storage compact={compact}")
     insts = small_instances(12, 123)
@@ -206,6 +207,8 @@ def test_batch_data_storage_forms(oracle_mod, compact):
     datas[2] = datas[2] + np.float32(0.25)
     datas[5] = datas[5].copy()
     datas[5][0, 1, 1] = 70000.0
+    datas[7] = datas[7].copy()
+    datas[7][-1, 2, 1] = 300.0
     bt = B.make_batch(insts, iters=5)
     bt.set_compact_data(compact)
     inits = [np.zeros((x["H"], x["W"]), np.int32) for x in insts]
