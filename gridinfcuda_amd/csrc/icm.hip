@@ -324,6 +324,198 @@ __global__ __launch_bounds__(256) void k_icm_async_lanes(const T *__restrict__ d
     if (active && q == 0) lab[n] = arg;
 }
 
+// ---- asynchronous ICM on tiles (L <= kLdsMaxLabels) -----------------------
+// The row-major in-place order is a wavefront over anti-diagonals; here the
+// wavefront runs over square tiles of TS x TS interior nodes.  Launch s
+// updates, for every iteration t with 0 <= d = s - 2t < tile diagonals, the
+// tiles (tx, ty) with tx + ty = d: a tile needs this iteration's labels of
+// its west and north tiles (diagonal d - 1, earlier launches) and the
+// previous iteration's of its east and south tiles (diagonal d + 1 of
+// iteration t - 1, launch s - 1; their iteration t comes at launch s + 1),
+// and no two tiles of one launch touch each other.  A block stages the
+// table in LDS (as synchronous ICM) and its tile's labels with a one-node
+// halo, then walks the tile's node anti-diagonals in order with a barrier
+// after each, 16 lanes per node as k_icm_async_lanes (same sums, same
+// tie-breaking), and writes the tile back.  The costs (node-major copy) of
+// the diagonal kPrefetch steps ahead are loaded while a step runs.  One
+// launch per tile diagonal replaces one per node diagonal.
+#ifndef GINF_ICM_TILE
+#define GINF_ICM_TILE 32
+#endif
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations and meets the other waves, without the release/acquire fence of
+// __syncthreads(), which also waits for every global load in flight (here the
+// cost prefetches of the next steps).  The memory clobber keeps the compiler
+// from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+constexpr int kTileLanes = 16;
+constexpr int kTilePerLane = kLdsMaxLabels / kTileLanes;  // labels per lane (8)
+constexpr int kPrefetch = 4;
+// Tile steps per iteration of the step loop (a multiple of kPrefetch): the
+// compiler waits for every load in flight at the loop's head, so longer
+// iterations drain the prefetch less often.
+#ifndef GINF_ICM_STEPS_PER_ITER
+#define GINF_ICM_STEPS_PER_ITER 16
+#endif
+constexpr int kStepsPerIter = GINF_ICM_STEPS_PER_ITER;
+static_assert(kStepsPerIter % kPrefetch == 0, "steps per loop iteration: a multiple of the prefetch depth");
+
+// stage_table with every load of a thread issued before its LDS stores (one
+// memory round trip per block instead of one per element group), over a
+// 128-column index space (row f = idx / 128, label l = idx % 128, no division
+// by L): the tile kernel stages the table once per launch, so its staging
+// time counts.
+template <typename T, int NT>
+__device__ __forceinline__ void stage_table_batched(T *lds, const T *__restrict__ table, int L, int Lp) {
+    constexpr int PER = (kLdsMaxLabels * kLdsMaxLabels + NT - 1) / NT;
+    T v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {  // unconditional loads (clamped in-bounds), so they issue back to back
+        const int idx = threadIdx.x + k * NT, f = idx / kLdsMaxLabels, l = idx % kLdsMaxLabels;
+        v[k] = table[min(f, L - 1) * L + min(l, L - 1)];  // table[l + L * f] = V(l, f)
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int idx = threadIdx.x + k * NT, f = idx / kLdsMaxLabels, l = idx % kLdsMaxLabels;
+        if (f < L && l < L) lds[f * Lp + (l ^ (f & 31))] = v[k];
+    }
+    __syncthreads();
+}
+
+// (cost, label) pair of a lane's lowest-index minimum; combine keeps the
+// smaller cost, on a tie the lower label (-1: no label).  The rule is
+// commutative and associative, so any reduction order gives the same pair.
+template <typename T>
+__device__ __forceinline__ void icm_combine(T &best, int &arg, T ob, int oa) {
+    if (oa >= 0 && (arg < 0 || ob < best || (!(best < ob) && oa < arg))) { best = ob; arg = oa; }
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_mov(T v) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+template <typename T, int TS>
+__global__ __launch_bounds__(TS * kTileLanes) void k_icm_async_tiles(const T *__restrict__ nm,
+                                                                     const T *__restrict__ table, int W, int H, int L,
+                                                                     int32_t *__restrict__ lab, int s, int t0,
+                                                                     int ntx, int nty, uint32_t nm_bytes) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char icm_smem[];
+    const int t = t0 + blockIdx.y;
+    const int d = s - 2 * t;
+    if (d < 0 || d >= ntx + nty - 1) return;  // uniform per block
+    const int tx = max(0, d - (nty - 1)) + blockIdx.x;
+    if (tx > min(ntx - 1, d)) return;
+    const int ty = d - tx;
+    const int x0 = 1 + tx * TS, y0 = 1 + ty * TS;
+    const int tw = min(TS, W - 1 - x0), th = min(TS, H - 1 - y0);  // tile extent inside the interior
+    constexpr int P = TS + 2;
+    constexpr int Lp = kLdsMaxLabels;  // table rows of 128 entries: l ^ (f & 31) stays in its row for any l < 128
+    __shared__ int ls[P * P];          // the tile's labels with a one-node halo
+    T *tab = (T *)icm_smem;            // the table, L rows of Lp (<= 64 KiB)
+    {
+        // the tile's labels and halo: unconditional loads (clamped in-bounds) issued together
+        constexpr int PL = (P * P + TS * kTileLanes - 1) / (TS * kTileLanes);
+        int v[PL];
+#pragma unroll
+        for (int k = 0; k < PL; ++k) {
+            const int i = threadIdx.x + k * TS * kTileLanes;
+            const int gy = min(y0 - 1 + i / P, H - 1), gx = min(x0 - 1 + i % P, W - 1);
+            v[k] = lab[(int64_t)gy * W + gx];
+        }
+#pragma unroll
+        for (int k = 0; k < PL; ++k) {
+            const int i = threadIdx.x + k * TS * kTileLanes;
+            if (i < P * P) ls[i] = clamp_label(v[k], L);  // entries past the image are never read
+        }
+    }
+    stage_table_batched<T, TS * kTileLanes>(tab, table, L, Lp);  // ends with a barrier
+    const int i = threadIdx.x / kTileLanes, q = threadIdx.x % kTileLanes;
+    const int nsteps = tw + th - 1;
+    // node of this thread at tile step k (lx < 0: none)
+    auto node_at = [&](int k, int &lx, int &ly) {
+        const int lo = max(0, k - (th - 1)), hi = min(tw - 1, k);
+        lx = lo + i;
+        ly = k - lx;
+        if (k >= nsteps || lx > hi) lx = -1;
+    };
+    // Costs through a buffer resource with 32-bit offsets: a lane's labels
+    // q, q + 16, ... of a node are kTileLanes * 4 B apart (immediate offsets);
+    // labels past L - 1 read the next node's costs (or 0 past the copy's end)
+    // and are not compared.
+    const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc((void *)nm, 0, nm_bytes, 0x00020000);
+    // Straight-line steps (no branch around a load, every prefetch issued,
+    // steps padded to a multiple of kStepsPerIter), so that the compiler can
+    // count the loads in flight and wait only for the slot a step uses.
+    T buf[kPrefetch][kTilePerLane];
+    auto load = [&](int k, T (&dv)[kTilePerLane]) {
+        int lx, ly;
+        node_at(k, lx, ly);
+        if (lx < 0) lx = ly = 0;  // no node: a harmless load of the tile's first node
+        const uint32_t off = ((uint32_t)((y0 + ly) * W + (x0 + lx)) * (uint32_t)L + (uint32_t)q) * (uint32_t)sizeof(T);
+#pragma unroll
+        for (int u = 0; u < kTilePerLane; ++u)
+            dv[u] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(crs, off + u * kTileLanes * sizeof(T), 0, 0));
+    };
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) load(u, buf[u]);
+    for (int k0 = 0; k0 < nsteps; k0 += kStepsPerIter) {
+#pragma unroll
+        for (int uu = 0; uu < kStepsPerIter; ++uu) {
+            const int k = k0 + uu, u = uu % kPrefetch;
+            int lx, ly;
+            node_at(k, lx, ly);
+            const bool act = lx >= 0;
+            const int c0 = act ? (ly + 1) * P + (lx + 1) : P + 1;
+            // waves without a node on this diagonal skip the step's work (a
+            // uniform branch with no loads inside; the prefetch below is issued
+            // by every wave)
+            const int span = min(tw - 1, k) - max(0, k - (th - 1));  // nodes on the diagonal - 1
+            const int wave_first = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64)) * (64 / kTileLanes);
+            T best = T(0);
+            int arg = -1;
+            if (k < nsteps && wave_first <= span) {
+                const int f4[4] = {ls[c0 + P], ls[c0 + 1], ls[c0 - P], ls[c0 - 1]};  // S, E, N, W
+                // row f, label l = q + 16 v sits at f * Lp + (l ^ (f & 31)) =
+                // f * Lp + (q ^ (f & 15)) + 16 * (v ^ ((f >> 4) & 1)): one base per
+                // row and parity of v, the rest immediate offsets
+                const T *be[4], *bo[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int f = f4[r];
+                    const T *row = tab + f * Lp + (q ^ (f & 15));
+                    const int sb = (f >> 4) & 1;
+                    be[r] = row + 16 * sb;   // even v: 16 * (v + sb)
+                    bo[r] = row - 16 * sb;   // odd v: 16 * (v - sb)
+                }
+#pragma unroll
+                for (int v = 0; v < kTilePerLane; ++v) {
+                    const int l = q + v * kTileLanes;
+                    T c = buf[u][v];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) c = c + ((v & 1) ? bo[r] : be[r])[16 * v];
+                    const bool take = act && l < L && (arg < 0 || c < best);
+                    best = take ? c : best;
+                    arg = take ? l : arg;
+                }
+                // reduce over the node's 16 lanes: lanes 1 and 2 of each quad,
+                // the two quads of each half row, the two halves of the row
+                icm_combine(best, arg, dpp_mov<0xB1>(best), dpp_mov<0xB1>(arg));    // quad_perm [1,0,3,2]
+                icm_combine(best, arg, dpp_mov<0x4E>(best), dpp_mov<0x4E>(arg));    // quad_perm [2,3,0,1]
+                icm_combine(best, arg, dpp_mov<0x141>(best), dpp_mov<0x141>(arg));  // row_half_mirror
+                icm_combine(best, arg, dpp_mov<0x140>(best), dpp_mov<0x140>(arg));  // row_mirror
+                if (act && q == 0) ls[c0] = arg;
+            }
+            load(k + kPrefetch, buf[u]);  // this slot's next diagonal
+            lds_barrier();
+        }
+    }
+    for (int j = threadIdx.x; j < tw * th; j += blockDim.x) {
+        const int ly = j / tw, lx = j % tw;
+        lab[(int64_t)(y0 + ly) * W + (x0 + lx)] = ls[(ly + 1) * P + (lx + 1)];
+    }
+}
+
 // Node-major copy of the costs for asynchronous ICM: out[n * L + l] =
 // data[l * P + n] (P = W * H), through a 32-label x 64-node LDS tile so that
 // both the plane reads and the node-row writes are contiguous.
@@ -462,6 +654,26 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
     hipLaunchKernelGGL(k_costs_node_major<T>, dim3((unsigned)((P + 63) / 64), (unsigned)((L + 31) / 32)), dim3(256),
                        0, st, data, P, L, nm);
     e = hipGetLastError();
+    const size_t nm_bytes = (size_t)P * L * sizeof(T);
+    if (lds && nm_bytes < ((size_t)1 << 32)) {
+        // tiles: one launch per tile anti-diagonal and pipelined iteration
+        constexpr int TS = GINF_ICM_TILE;
+        const size_t tile_lds = (size_t)L * kLdsMaxLabels * sizeof(T);
+        const int ntx = (W - 2 + TS - 1) / TS, nty = (H - 2 + TS - 1) / TS, ndt = ntx + nty - 1;
+        const int tsteps = ndt + 2 * (iters - 1);
+        const dim3 block(TS * kTileLanes);
+        for (int s = 0; s < tsteps && e == hipSuccess; ++s) {
+            const int t_lo = s < ndt ? 0 : (s - ndt) / 2 + 1;
+            const int t_hi = min(iters - 1, s / 2);
+            if (t_lo > t_hi) continue;
+            const dim3 grid((unsigned)min(ntx, nty), (unsigned)(t_hi - t_lo + 1));
+            hipLaunchKernelGGL((k_icm_async_tiles<T, TS>), grid, block, tile_lds, st, nm, table, W, H, L, result, s,
+                               t_lo, ntx, nty, (uint32_t)nm_bytes);
+            e = hipGetLastError();
+        }
+        const hipError_t f = hipFreeAsync(nm, st);
+        return e != hipSuccess ? e : f;
+    }
     for (int s = 0; s < steps && e == hipSuccess; ++s) {
         // iterations with 0 <= s - 2t < nd
         const int t_lo = s < nd ? 0 : (s - nd) / 2 + 1;
