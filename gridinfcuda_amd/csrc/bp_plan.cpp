@@ -296,7 +296,7 @@ int64_t interior_of_color(int w, int h, int color) {
 
 // The storage form of the last run's data costs (waits for the device): 0 T,
 // 1 16-bit, 2 8-bit at levels 0 and 1 and 16-bit above.
-hipError_t int16_used(ginf_bp_plan p, int *used) {
+hipError_t data_form_used(ginf_bp_plan p, int *used) {
     *used = 0;
     if (!p->compact_last || !p->flag_dev) return hipSuccess;
     int f[2] = {1, 1};
@@ -758,7 +758,7 @@ int ginf_bp_plan_pairs(ginf_bp_plan plan, int *enabled) {
 
 int ginf_bp_plan_compact_data(ginf_bp_plan plan, int *last_run_compact) {
     if (!plan || !last_run_compact) return set_error(GINF_ERR_INVALID, "NULL argument");
-    int16_used(plan, last_run_compact);
+    GINF_HIP_TRY(data_form_used(plan, last_run_compact));
     return GINF_OK;
 }
 
@@ -940,7 +940,7 @@ int ginf_bp_plan_dump_data(ginf_bp_plan p, int level, void *data_host) {
     std::vector<uint32_t> buf(n);
     GINF_HIP_TRY(hipDeviceSynchronize());
     int form = 0;
-    GINF_HIP_TRY(int16_used(p, &form));
+    GINF_HIP_TRY(data_form_used(p, &form));
     const int bytes = form == 0 ? 4 : (form == 2 && level <= 1) ? 1 : 2;
     std::vector<uint16_t> buf16(bytes == 2 ? n : 0);
     std::vector<uint8_t> buf8(bytes == 1 ? n : 0);
