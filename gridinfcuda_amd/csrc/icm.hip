@@ -384,8 +384,9 @@ __device__ __forceinline__ void stage_table_batched(T *lds, const T *__restrict_
 }
 
 // (cost, label) pair of a lane's lowest-index minimum; combine keeps the
-// smaller cost, on a tie the lower label (-1: no label).  The rule is
-// commutative and associative, so any reduction order gives the same pair.
+// smaller cost, on a tie the lower label (-1: no label).  For costs that are
+// not NaN the rule is commutative and associative, so any reduction order
+// gives the same pair: the sequential scan's.
 template <typename T>
 __device__ __forceinline__ void icm_combine(T &best, int &arg, T ob, int oa) {
     if (oa >= 0 && (arg < 0 || ob < best || (!(best < ob) && oa < arg))) { best = ob; arg = oa; }
