@@ -271,9 +271,9 @@ def bench_icm(args) -> dict:
     decodeMaxPrior labelling, 10 iterations, device-resident, HIP events on
     the launch stream (median of the timed runs).  Metric: node-label
     evaluations/s (interior nodes x labels x iterations / time).  The
-    synchronous kernel reads every interior node's L costs once per iteration
-    (4 B per node-label) plus its label and writes the new one (8 B per node):
-    HBM-bound.  The asynchronous mode (the reference's in-place row-major
+    synchronous call's first iteration reads the costs in T and writes an
+    8-bit copy of them, the later ones read the copy (1 B per node-label plus
+    8 B of labels per node).  The asynchronous mode (the reference's in-place row-major
     order) runs as a wavefront over 32 x 32 tiles (one launch per tile
     anti-diagonal and pipelined iteration) and is latency-bound.  CPU:
     the oracle, one thread, on the same recipe at 960x540 (a quarter of the
@@ -315,14 +315,20 @@ def bench_icm(args) -> dict:
     sync_ms, data, init, _ = gpu_run(W, H, G.ICM_SYNC, runs)
     async_ms, _, _, _ = gpu_run(W, H, G.ICM_ASYNC, max(3, runs // 2))
     del data, init
-    algo = (W - 2) * (H - 2) * (4 * L + 8) * N
-    # HBM bytes per synchronous iteration, measured with rocprofv3 FETCH_SIZE /
+    # Algorithmic bytes of one synchronous call: the first iteration reads the
+    # costs in T (4 B per interior node-label), writes their 8-bit copy (1 B)
+    # and the labels (8 B per node); the other N - 1 read the copy (1 B per
+    # node-label + 8 B per node).  The stereo costs are integers in [0, 40],
+    # so the copy holds them all.
+    nin = (W - 2) * (H - 2)
+    algo = nin * (4 * L + L + 8) + (N - 1) * nin * (L + 8)
+    # HBM bytes per synchronous call, measured with rocprofv3 FETCH_SIZE /
     # WRITE_SIZE (scaled by a 1 GiB copy) on this workload; see the profile summary.
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "icm_sync_1920x1080x128_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f)["traffic_bytes_per_launch"]
+            traffic = json.load(f)["traffic_bytes_per_call"]
     # CPU sample and its parity check
     Ws, Hs = 960, 540
     _, sd, sinit, sout = gpu_run(Ws, Hs, G.ICM_SYNC, 1)
@@ -347,7 +353,9 @@ def bench_icm(args) -> dict:
                    "labels_equal_oracle_at_960x540": same},
         "roofline": {"bound": "hbm", "achieved": round(algo / sync_ms / 1e6, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(algo / sync_ms / 1e6 / 8000.0, 4), "traffic": traffic,
-                     "kernel": "synchronous ICM, 10 launches (4 B per node-label + 8 B per node and iteration)"},
+                     "kernel": "synchronous ICM call: the first of 10 launches reads the costs in T and writes "
+                               "their 8-bit copy, the others read the copy (1 B per node-label + 8 B per node); "
+                               "LDS- and VALU-limited since the copy; traffic per call"},
         "cpu_baseline": {"value": round(cpu_evals / cpu_s / 1e9, 5), "unit": "Gnode-label evaluations/s", "cores": 1,
                          "kind": "port", "sample": "the same recipe at 960x540, 10 synchronous iterations, one run",
                          "seconds": round(cpu_s, 3), "cpu_model": cpu_model()},

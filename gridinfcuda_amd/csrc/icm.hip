@@ -23,11 +23,50 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "bp_kernels.h"
 #include "ginf_bp.h"
 
 namespace {
+
+// Stream-ordered scratch allocations (label buffer, 8-bit cost copy,
+// node-major cost copy) come from a memory pool of the library's own per
+// device, which keeps freed memory for the next call (release threshold at
+// its maximum) instead of unmapping it at every synchronisation; the process's
+// default pool is left as it is.  Pool memory lives until the process ends.
+// Returns nullptr when the pool cannot be created (the caller then uses the
+// default pool).
+hipMemPool_t scratch_pool(int dev) {
+    constexpr int kMaxDev = 64;
+    static hipMemPool_t pools[kMaxDev] = {};
+    static std::mutex mu;
+    if (dev < 0 || dev >= kMaxDev) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!pools[dev]) {
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        if (hipMemPoolCreate(&pool, &props) != hipSuccess) return nullptr;
+        uint64_t keep = UINT64_MAX;
+        if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+            (void)hipMemPoolDestroy(pool);
+            return nullptr;
+        }
+        pools[dev] = pool;
+    }
+    return pools[dev];
+}
+
+hipError_t scratch_alloc(void **p, size_t n, hipStream_t st) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    hipMemPool_t pool = scratch_pool(dev);
+    return pool ? hipMallocFromPoolAsync(p, n, pool, st) : hipMallocAsync(p, n, st);
+}
 
 // Neighbour labels outside [0, L) (undefined in the reference) are clamped so
 // that the table reads stay inside the table.
@@ -180,14 +219,45 @@ __device__ __forceinline__ T lane_of(const typename Vec4<T>::type &v, int k) {
     return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
 
+// Four nodes' costs of one label: a 16-byte load of T, or a 4-byte load of
+// the 8-bit copy (exact: every cost is an integer in [0, 255]).
+template <typename T, typename S>
+__device__ __forceinline__ void load_cost4(const S *p, T (&c)[4]) {
+    if constexpr (sizeof(S) == 1) {
+        const uint32_t v = *reinterpret_cast<const uint32_t *>(p);
+        c[0] = T(v & 0xFFu);
+        c[1] = T((v >> 8) & 0xFFu);
+        c[2] = T((v >> 16) & 0xFFu);
+        c[3] = T(v >> 24);
+    } else {
+        const typename Vec4<T>::type v = *reinterpret_cast<const typename Vec4<T>::type *>(p);
+        c[0] = v.x;
+        c[1] = v.y;
+        c[2] = v.z;
+        c[3] = v.w;
+    }
+}
+
+// Exact 8-bit form of a cost: an integer in [0, 255] (and not -0).
 template <typename T>
-__global__ __launch_bounds__(768) void k_icm_sync_lds_x4(const T *__restrict__ data, const T *__restrict__ table,
-                                                          int W, int H, int L, const int32_t *__restrict__ old,
-                                                          int32_t *__restrict__ out) {
+__device__ __forceinline__ bool fits_u8(T e) {
+    if constexpr (std::is_same<T, float>::value)
+        return e >= 0.f && e <= 255.f && e == __builtin_truncf(e) && !__builtin_signbit(e);
+    else
+        return e >= 0 && e <= 255;
+}
+
+// One synchronous iteration over the interior, four nodes per thread
+// (grid-stride), the table already in LDS.  S = T reads the costs in T; S =
+// uint8_t reads their 8-bit copy.  COPY (with S = T): also writes the 8-bit
+// copy of every cost it reads into `copy` (same [l][y][x] layout) and sets
+// *bad for a cost with no exact 8-bit form.
+template <typename T, typename S, bool COPY>
+__device__ __forceinline__ void icm_x4_pass(const S *__restrict__ data, const unsigned char *tab_raw, int W, int H, int L,
+                                            const int32_t *__restrict__ old, int32_t *__restrict__ out,
+                                            uint8_t *__restrict__ copy, bool &bad) {
     using V4 = typename Vec4<T>::type;
-    extern __shared__ unsigned char smem_raw[];
     const int Lr = quad_row_stride(L);
-    stage_table_quads(reinterpret_cast<T *>(smem_raw), table, L, Lr);
     const int qw = W >> 2;
     const int64_t nq = (int64_t)qw * (H - 2), plane = (int64_t)W * H;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
@@ -216,23 +286,35 @@ __global__ __launch_bounds__(768) void k_icm_sync_lds_x4(const T *__restrict__ d
         // issued before the current quad's sums, so every wave keeps loads in
         // flight while it computes (the last quad re-reads plane L-1 instead
         // of branching).
-        V4 dv[4], nx[4];
+        T dv[4][4], nx[4][4];  // [label l0 + u][node k]
 #pragma unroll
-        for (int u = 0; u < 4; ++u) dv[u] = *reinterpret_cast<const V4 *>(data + (int64_t)u * plane + n0);
+        for (int u = 0; u < 4; ++u) load_cost4(data + (int64_t)u * plane + n0, dv[u]);
         for (int l0 = 0; l0 < L; l0 += 4) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                nx[u] = *reinterpret_cast<const V4 *>(data + (int64_t)min(l0 + 4 + u, L - 1) * plane + n0);
+            for (int u = 0; u < 4; ++u) load_cost4(data + (int64_t)min(l0 + 4 + u, L - 1) * plane + n0, nx[u]);
+            if constexpr (COPY) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    uint32_t o = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const bool ok = fits_u8(dv[u][k]);
+                        bad |= !ok;
+                        o |= (ok ? (uint32_t)dv[u][k] : 0u) << (8 * k);
+                    }
+                    *reinterpret_cast<uint32_t *>(copy + (int64_t)(l0 + u) * plane + n0) = o;
+                }
+            }
             const int lb = l0 * (int)sizeof(T);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 V4 t[4];  // V(l0..l0+3, f) for the S, E, N, W neighbours of node k
 #pragma unroll
-                for (int d = 0; d < 4; ++d) t[d] = *reinterpret_cast<const V4 *>(smem_raw + (P[k][d] ^ lb));
+                for (int d = 0; d < 4; ++d) t[d] = *reinterpret_cast<const V4 *>(tab_raw + (P[k][d] ^ lb));
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int l = l0 + u;
-                    T c = lane_of<T>(dv[u], k);
+                    T c = dv[u][k];
                     c = c + lane_of<T>(t[0], u);
                     c = c + lane_of<T>(t[1], u);
                     c = c + lane_of<T>(t[2], u);
@@ -241,11 +323,42 @@ __global__ __launch_bounds__(768) void k_icm_sync_lds_x4(const T *__restrict__ d
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) dv[u] = nx[u];
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) dv[u][k] = nx[u][k];
         }
         const bool xl = x0 == 0, xr = x0 + 4 == W;  // border columns keep their label
         *reinterpret_cast<int4 *>(out + n0) =
             int4{xl ? cur[0] : arg[0], arg[1], arg[2], xr ? cur[3] : arg[3]};
+    }
+}
+
+// Synchronous iteration with four nodes per thread.  MODE 0 reads the costs
+// in T; MODE 1 (the first iteration of a call) reads them in T and writes
+// their 8-bit copy `c8` and *flag (set when a cost has no exact 8-bit form);
+// MODE 2 (the later iterations) reads the copy when *flag is clear, else the
+// costs in T -- a quarter of the cost bytes per iteration for stereo-style
+// costs, decided on the device.  The copy holds every node of every quad an
+// iteration reads (rows 1..H-2); a border column's costs are read and
+// discarded.
+template <typename T, int MODE>
+__global__ __launch_bounds__(768) void k_icm_sync_lds_x4(const T *__restrict__ data, uint8_t *__restrict__ c8,
+                                                          int *__restrict__ flag, const T *__restrict__ table, int W,
+                                                          int H, int L, const int32_t *__restrict__ old,
+                                                          int32_t *__restrict__ out) {
+    extern __shared__ unsigned char smem_raw[];
+    stage_table_quads(reinterpret_cast<T *>(smem_raw), table, L, quad_row_stride(L));
+    bool bad = false;
+    if constexpr (MODE == 2) {
+        if (__builtin_amdgcn_readfirstlane(*flag) == 0)
+            icm_x4_pass<T, uint8_t, false>(c8, smem_raw, W, H, L, old, out, nullptr, bad);
+        else
+            icm_x4_pass<T, T, false>(data, smem_raw, W, H, L, old, out, nullptr, bad);
+    } else if constexpr (MODE == 1) {
+        icm_x4_pass<T, T, true>(data, smem_raw, W, H, L, old, out, c8, bad);
+        if (bad) atomicOr(flag, 1);
+    } else {
+        icm_x4_pass<T, T, false>(data, smem_raw, W, H, L, old, out, nullptr, bad);
     }
 }
 
@@ -613,7 +726,7 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
     const size_t lds_bytes_x4 = (size_t)L * quad_row_stride(L) * sizeof(T);
     if (mode == GINF_ICM_SYNC) {
         int32_t *tmp = nullptr;
-        if ((e = hipMallocAsync((void **)&tmp, nl, st)) != hipSuccess) return e;
+        if ((e = scratch_alloc((void **)&tmp, nl, st)) != hipSuccess) return e;
         e = hipMemcpyAsync(tmp, initial, nl, hipMemcpyDeviceToDevice, st);
         // The last iteration writes `result`.
         int32_t *bufs[2] = {(iters & 1) ? tmp : result, (iters & 1) ? result : tmp};
@@ -630,11 +743,29 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
         const int bs = x4 ? 768 : 1024;
         const unsigned lds_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + bs - 1) / bs, 2 * (int64_t)cus));
         const dim3 grid((unsigned)((W - 2 + 255) / 256), (unsigned)(H - 2)), block(256);
+        // With four nodes per thread the first iteration also writes an 8-bit
+        // copy of the costs (and a flag for costs without an exact 8-bit
+        // form), and the later ones read the copy where it holds every cost.
+        uint8_t *d8 = nullptr;
+        int *flag = nullptr;
+        const bool c8 = x4 && iters > 1;
+        if (c8 && e == hipSuccess) {
+            if ((e = scratch_alloc((void **)&d8, (size_t)W * H * L, st)) == hipSuccess &&
+                (e = scratch_alloc((void **)&flag, sizeof(int), st)) == hipSuccess)
+                e = ginf::launch_fill_u32(flag, 1, 0u, st);
+        }
         for (int t = 0; t < iters && e == hipSuccess; ++t) {
-            if (x4)
-                hipLaunchKernelGGL(k_icm_sync_lds_x4<T>, dim3(lds_grid), dim3(bs), lds_bytes_x4, st, data, table, W, H,
-                                   L, bufs[t & 1], bufs[(t + 1) & 1]);
-            else if (lds)
+            if (x4) {
+                if (!c8)
+                    hipLaunchKernelGGL((k_icm_sync_lds_x4<T, 0>), dim3(lds_grid), dim3(bs), lds_bytes_x4, st, data, d8,
+                                       flag, table, W, H, L, bufs[t & 1], bufs[(t + 1) & 1]);
+                else if (t == 0)
+                    hipLaunchKernelGGL((k_icm_sync_lds_x4<T, 1>), dim3(lds_grid), dim3(bs), lds_bytes_x4, st, data, d8,
+                                       flag, table, W, H, L, bufs[t & 1], bufs[(t + 1) & 1]);
+                else
+                    hipLaunchKernelGGL((k_icm_sync_lds_x4<T, 2>), dim3(lds_grid), dim3(bs), lds_bytes_x4, st, data, d8,
+                                       flag, table, W, H, L, bufs[t & 1], bufs[(t + 1) & 1]);
+            } else if (lds)
                 hipLaunchKernelGGL(k_icm_sync_lds<T>, dim3(lds_grid), dim3(1024), lds_bytes, st, data, table, W, H, L,
                                    bufs[t & 1], bufs[(t + 1) & 1]);
             else
@@ -642,7 +773,9 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
                                    bufs[(t + 1) & 1]);
             e = hipGetLastError();
         }
-        const hipError_t f = hipFreeAsync(tmp, st);
+        hipError_t f = hipFreeAsync(tmp, st);
+        if (d8 && f == hipSuccess) f = hipFreeAsync(d8, st);
+        if (flag && f == hipSuccess) f = hipFreeAsync(flag, st);
         return e != hipSuccess ? e : f;
     }
     const int nd = (W - 2) + (H - 2) - 1;  // anti-diagonals of the interior
@@ -651,7 +784,7 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
     // Node-major copy of the costs (one read and one write of the volume).
     const int64_t P = (int64_t)W * H;
     T *nm = nullptr;
-    if ((e = hipMallocAsync((void **)&nm, (size_t)P * L * sizeof(T), st)) != hipSuccess) return e;
+    if ((e = scratch_alloc((void **)&nm, (size_t)P * L * sizeof(T), st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_costs_node_major<T>, dim3((unsigned)((P + 63) / 64), (unsigned)((L + 31) / 32)), dim3(256),
                        0, st, data, P, L, nm);
     e = hipGetLastError();

@@ -56,6 +56,32 @@ def test_icm_matches_oracle(oracle_mod, dtype, mode, L, H, W, iters, kind):
     np.testing.assert_array_equal(G.icm_host(data, table, init, iters, mode), want)
 
 
+@pytest.mark.parametrize("form", ["fits", "past255", "fraction", "negzero"])
+def test_icm_sync_cost_storage(oracle_mod, form):
+    """Synchronous ICM with four nodes per thread reads an 8-bit copy of the
+    costs when every cost is an integer in [0, 255], else the costs
+    themselves (decided on the device); either way the labels equal the
+    oracle's."""
+    torch = torch_cuda()
+    L, H, W, iters = 16, 29, 48, 4
+    dtype = np.int32 if form == "past255" else np.float32
+    say(f"icm sync cost storage {form}")
+    data, table, init = case(11, L, H, W, dtype, "tl")
+    if form == "past255":
+        data[3, 7, 9] = 300
+    elif form == "fraction":
+        data[5, 10, 20] += 0.5
+    elif form == "negzero":
+        data[0, 4, 4] = -0.0
+    want = oracle_mod.icm(data, table, init, iters, sync=True)
+    dd, td, idev = (torch.from_numpy(a).cuda() for a in (data, table, init))
+    out = torch.full((H, W), -1, dtype=torch.int32, device="cuda")
+    G.icm(W, H, L, G.dtype_code(data.dtype), dd.data_ptr(), td.data_ptr(), idev.data_ptr(), iters, G.ICM_SYNC,
+          out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), want)
+
+
 @pytest.mark.parametrize("dtype", [np.int32, np.float32])
 def test_icm_sync_unaligned_buffers(oracle_mod, dtype):
     """W % 4 == 0 but a result buffer 4 bytes past a 16-byte boundary: the
