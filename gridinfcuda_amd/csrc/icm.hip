@@ -462,8 +462,11 @@ __global__ __launch_bounds__(256) void k_icm_async_lanes(const T *__restrict__ d
 // cost prefetches of the next steps).  The memory clobber keeps the compiler
 // from moving LDS accesses across it.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-constexpr int kTileLanes = 16;
-constexpr int kTilePerLane = kLdsMaxLabels / kTileLanes;  // labels per lane (8)
+#ifndef GINF_ICM_TILE_LANES
+#define GINF_ICM_TILE_LANES 16
+#endif
+constexpr int kTileLanes = GINF_ICM_TILE_LANES;  // lanes per node (8 or 16)
+static_assert(kTileLanes == 8 || kTileLanes == 16, "8 or 16 lanes per node");
 constexpr int kPrefetch = 4;
 // Tile steps per iteration of the step loop (a multiple of kPrefetch): the
 // compiler waits for every load in flight at the loop's head, so longer
@@ -509,8 +512,8 @@ __device__ __forceinline__ T dpp_mov(T v) {
     return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
 }
 
-template <typename T, int TS>
-__global__ __launch_bounds__(TS * kTileLanes) void k_icm_async_tiles(const T *__restrict__ nm,
+template <typename T, int TS, int NL>
+__global__ __launch_bounds__(TS * NL) void k_icm_async_tiles(const T *__restrict__ nm,
                                                                      const T *__restrict__ table, int W, int H, int L,
                                                                      int32_t *__restrict__ lab, int s, int t0,
                                                                      int ntx, int nty, uint32_t nm_bytes) {
@@ -529,22 +532,23 @@ __global__ __launch_bounds__(TS * kTileLanes) void k_icm_async_tiles(const T *__
     T *tab = (T *)icm_smem;            // the table, L rows of Lp (<= 64 KiB)
     {
         // the tile's labels and halo: unconditional loads (clamped in-bounds) issued together
-        constexpr int PL = (P * P + TS * kTileLanes - 1) / (TS * kTileLanes);
+        constexpr int PL = (P * P + TS * NL - 1) / (TS * NL);
         int v[PL];
 #pragma unroll
         for (int k = 0; k < PL; ++k) {
-            const int i = threadIdx.x + k * TS * kTileLanes;
+            const int i = threadIdx.x + k * TS * NL;
             const int gy = min(y0 - 1 + i / P, H - 1), gx = min(x0 - 1 + i % P, W - 1);
             v[k] = lab[(int64_t)gy * W + gx];
         }
 #pragma unroll
         for (int k = 0; k < PL; ++k) {
-            const int i = threadIdx.x + k * TS * kTileLanes;
+            const int i = threadIdx.x + k * TS * NL;
             if (i < P * P) ls[i] = clamp_label(v[k], L);  // entries past the image are never read
         }
     }
-    stage_table_batched<T, TS * kTileLanes>(tab, table, L, Lp);  // ends with a barrier
-    const int i = threadIdx.x / kTileLanes, q = threadIdx.x % kTileLanes;
+    stage_table_batched<T, TS * NL>(tab, table, L, Lp);  // ends with a barrier
+    constexpr int PER = kLdsMaxLabels / NL;  // labels per lane
+    const int i = threadIdx.x / NL, q = threadIdx.x % NL;
     const int nsteps = tw + th - 1;
     // node of this thread at tile step k (lx < 0: none)
     auto node_at = [&](int k, int &lx, int &ly) {
@@ -554,22 +558,22 @@ __global__ __launch_bounds__(TS * kTileLanes) void k_icm_async_tiles(const T *__
         if (k >= nsteps || lx > hi) lx = -1;
     };
     // Costs through a buffer resource with 32-bit offsets: a lane's labels
-    // q, q + 16, ... of a node are kTileLanes * 4 B apart (immediate offsets);
+    // q, q + NL, ... of a node are NL * sizeof(T) bytes apart (immediate offsets);
     // labels past L - 1 read the next node's costs (or 0 past the copy's end)
     // and are not compared.
     const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc((void *)nm, 0, nm_bytes, 0x00020000);
     // Straight-line steps (no branch around a load, every prefetch issued,
     // steps padded to a multiple of kStepsPerIter), so that the compiler can
     // count the loads in flight and wait only for the slot a step uses.
-    T buf[kPrefetch][kTilePerLane];
-    auto load = [&](int k, T (&dv)[kTilePerLane]) {
+    T buf[kPrefetch][PER];
+    auto load = [&](int k, T (&dv)[PER]) {
         int lx, ly;
         node_at(k, lx, ly);
         if (lx < 0) lx = ly = 0;  // no node: a harmless load of the tile's first node
         const uint32_t off = ((uint32_t)((y0 + ly) * W + (x0 + lx)) * (uint32_t)L + (uint32_t)q) * (uint32_t)sizeof(T);
 #pragma unroll
-        for (int u = 0; u < kTilePerLane; ++u)
-            dv[u] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(crs, off + u * kTileLanes * sizeof(T), 0, 0));
+        for (int u = 0; u < PER; ++u)
+            dv[u] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(crs, off + u * NL * sizeof(T), 0, 0));
     };
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) load(u, buf[u]);
@@ -585,39 +589,41 @@ __global__ __launch_bounds__(TS * kTileLanes) void k_icm_async_tiles(const T *__
             // uniform branch with no loads inside; the prefetch below is issued
             // by every wave)
             const int span = min(tw - 1, k) - max(0, k - (th - 1));  // nodes on the diagonal - 1
-            const int wave_first = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64)) * (64 / kTileLanes);
+            const int wave_first = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64)) * (64 / NL);
             T best = T(0);
             int arg = -1;
             if (k < nsteps && wave_first <= span) {
                 const int f4[4] = {ls[c0 + P], ls[c0 + 1], ls[c0 - P], ls[c0 - 1]};  // S, E, N, W
-                // row f, label l = q + 16 v sits at f * Lp + (l ^ (f & 31)) =
-                // f * Lp + (q ^ (f & 15)) + 16 * (v ^ ((f >> 4) & 1)): one base per
-                // row and parity of v, the rest immediate offsets
-                const T *be[4], *bo[4];
+                // row f, label l = q + NL v sits at f * Lp + (l ^ (f & 31)) =
+                // f * Lp + (q ^ (f & (NL - 1))) + NL * (v ^ sb), sb = (f & 31) / NL:
+                // one base per row and value of v mod G (the low bits sb can
+                // flip), the rest immediate offsets
+                constexpr int G = 32 / NL;
+                const T *base[4][G];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int f = f4[r];
-                    const T *row = tab + f * Lp + (q ^ (f & 15));
-                    const int sb = (f >> 4) & 1;
-                    be[r] = row + 16 * sb;   // even v: 16 * (v + sb)
-                    bo[r] = row - 16 * sb;   // odd v: 16 * (v - sb)
+                    const T *row = tab + f * Lp + (q ^ (f & (NL - 1)));
+                    const int sb = (f & 31) / NL;
+#pragma unroll
+                    for (int j = 0; j < G; ++j) base[r][j] = row + NL * ((j ^ sb) - j);
                 }
 #pragma unroll
-                for (int v = 0; v < kTilePerLane; ++v) {
-                    const int l = q + v * kTileLanes;
+                for (int v = 0; v < PER; ++v) {
+                    const int l = q + v * NL;
                     T c = buf[u][v];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) c = c + ((v & 1) ? bo[r] : be[r])[16 * v];
+                    for (int r = 0; r < 4; ++r) c = c + base[r][v % G][NL * v];
                     const bool take = act && l < L && (arg < 0 || c < best);
                     best = take ? c : best;
                     arg = take ? l : arg;
                 }
-                // reduce over the node's 16 lanes: lanes 1 and 2 of each quad,
-                // the two quads of each half row, the two halves of the row
+                // reduce over the node's NL lanes: lanes 1 and 2 of each quad,
+                // the two quads of each half row, (NL = 16) the two halves of the row
                 icm_combine(best, arg, dpp_mov<0xB1>(best), dpp_mov<0xB1>(arg));    // quad_perm [1,0,3,2]
                 icm_combine(best, arg, dpp_mov<0x4E>(best), dpp_mov<0x4E>(arg));    // quad_perm [2,3,0,1]
                 icm_combine(best, arg, dpp_mov<0x141>(best), dpp_mov<0x141>(arg));  // row_half_mirror
-                icm_combine(best, arg, dpp_mov<0x140>(best), dpp_mov<0x140>(arg));  // row_mirror
+                if constexpr (NL == 16) icm_combine(best, arg, dpp_mov<0x140>(best), dpp_mov<0x140>(arg));  // row_mirror
                 if (act && q == 0) ls[c0] = arg;
             }
             load(k + kPrefetch, buf[u]);  // this slot's next diagonal
@@ -801,7 +807,7 @@ hipError_t icm_t(const T *data, const T *table, int W, int H, int L, const int32
             const int t_hi = min(iters - 1, s / 2);
             if (t_lo > t_hi) continue;
             const dim3 grid((unsigned)min(ntx, nty), (unsigned)(t_hi - t_lo + 1));
-            hipLaunchKernelGGL((k_icm_async_tiles<T, TS>), grid, block, tile_lds, st, nm, table, W, H, L, result, s,
+            hipLaunchKernelGGL((k_icm_async_tiles<T, TS, kTileLanes>), grid, block, tile_lds, st, nm, table, W, H, L, result, s,
                                t_lo, ntx, nty, (uint32_t)nm_bytes);
             e = hipGetLastError();
         }
