@@ -47,7 +47,12 @@ constexpr int pitch_in() { return CS * 64 + 16; }
 
 // One convolution over an output region of RH x RW pixels whose input region
 // (RH + 2) x (RW + 2) is in LDS at `in` (pitch PIN bytes, CS 32-channel
-// steps).  Output co tiles of 16 channels are dealt in chunks of CPC tiles:
+// steps).  The MFMA takes the weights as its first operand and the
+// activations as its second, so it yields the transposed output tile: lane
+// (m, kg) holds channels 4 kg .. 4 kg + 3 of pixel m, stored with one 8-byte
+// (hidden, BF16) or 16-byte (output, FP32) write.  Each product and the order
+// of the sums are those of the untransposed product.
+// Output co tiles of 16 channels are dealt in chunks of CPC tiles:
 // wave w takes chunk w % NCH and the pixel groups w / NCH, w / NCH + NW / NCH,
 // ...; a chunk's B fragments stay in registers while its waves walk their
 // pixel groups, and each A fragment read from LDS feeds CPC MFMAs.  The
@@ -65,7 +70,7 @@ struct Layer {
     static constexpr int GSTEP = kWaves / NCH;
     static constexpr int RWI = RW + 2;
     bf16x8 b[CPC][9][CS];
-    float bv[CPC];
+    float bv[CPC][4];  // biases of the lane's channels 4 kg .. 4 kg + 3 of each co tile
 
     // B fragments of this wave's co chunk straight from the weights in HBM/L2.
     __device__ __forceinline__ void load(const uint16_t *__restrict__ w, const float *__restrict__ bias) {
@@ -79,7 +84,8 @@ struct Layer {
             for (int t = 0; t < 9; ++t)
 #pragma unroll
                 for (int c = 0; c < CS; ++c) b[j][t][c] = load8(w + ((int64_t)co * 9 + t) * CIN + c * 32 + kg * 8);
-            bv[j] = bias[co];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bv[j][i] = bias[(chunk * CPC + j) * 16 + 4 * kg + i];
         }
     }
 
@@ -97,7 +103,8 @@ struct Layer {
 #pragma unroll
                 for (int c = 0; c < CS; ++c)
                     b[j][t][c] = *reinterpret_cast<const bf16x8 *>(wl + (co * R + t * (CIN / 8) + c * 4 + kg) * 16);
-            bv[j] = bias[co];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bv[j][i] = bias[(chunk * CPC + j) * 16 + 4 * kg + i];
         }
     }
 
@@ -128,28 +135,32 @@ struct Layer {
                 for (int c = 0; c < CS; ++c)
 #pragma unroll
                     for (int j = 0; j < CPC; ++j)
-                        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t][c], b[j][t][c], acc[j], 0, 0, 0);
+                        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j][t][c], a[t][c], acc[j], 0, 0, 0);
             // schedule: the group's LDS reads, then its MFMAs (left alone, the
             // scheduler issues two reads ahead and waits before every MFMA pair)
             __builtin_amdgcn_sched_group_barrier(0x100, 9 * CS, 0);
             __builtin_amdgcn_sched_group_barrier(0x008, 9 * CS * CPC, 0);
-            // lane holds rows 4 * kg + i (pixels) of column m (channel) of every tile
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int q = g * 16 + kg * 4 + i;
-                if (q >= NPIX) continue;
-                const int rr = q / RW, cc = q - (q / RW) * RW;
-                const int gy = oy + rr, gx = ox + cc;
+            // lane holds rows 4 * kg + i (channels) of column m (pixel p) of
+            // every tile; lanes past the region store nothing
+            if (p < NPIX) {
+                const int gy = oy + r, gx = ox + c0;
                 const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
 #pragma unroll
                 for (int j = 0; j < CPC; ++j) {
-                    const int co = (chunk * CPC + j) * 16 + m;
-                    float v = acc[j][i] + bv[j];
+                    const int co0 = (chunk * CPC + j) * 16 + 4 * kg;
+                    float v[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) v[i] = acc[j][i] + bv[j][i];
                     if constexpr (LAST) {
-                        if (inside) gout[(img_px0 + (int64_t)gy * W + gx) * (CT * 16) + co] = v;
+                        if (inside)
+                            *reinterpret_cast<f4 *>(gout + (img_px0 + (int64_t)gy * W + gx) * (CT * 16) + co0) =
+                                f4{v[0], v[1], v[2], v[3]};
                     } else {
-                        v = v > 0.f ? v : 0.f;
-                        *reinterpret_cast<uint16_t *>(out + q * kPitchHid + co * 2) = inside ? f2bf(v) : (uint16_t)0;
+                        uint32_t h[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) h[i] = inside ? f2bf(v[i] > 0.f ? v[i] : 0.f) : 0u;
+                        *reinterpret_cast<uint2 *>(out + p * kPitchHid + co0 * 2) =
+                            uint2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
                     }
                 }
             }
