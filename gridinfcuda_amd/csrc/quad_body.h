@@ -62,8 +62,17 @@ __device__ __forceinline__ int quad_bcast(int v) {
 //         outboxes ginf_bp_plan_seed would have stored, read where they come
 //         from (FP32 only; a lane's label quad is 16 contiguous bytes of the
 //         prediction row, L % 4 == 0).
+// LABELS with LMAX <= 128: the quad writes the node's beliefs to its row of
+// the block's LDS (`brow`, LMAX floats) inside the label loop, and
+// the quad takes the minimum from there after the outboxes are stored, when
+// the message registers are free; lanes hold no running minimum in the loop
+// (that kept the labelling instance at two waves per SIMD instead of three).
+// LMAX = 256 keeps the running per-lane minimum.
+template <bool LABELS, int LMAX>
+constexpr bool kLdsLabels = LABELS && LMAX <= 128 && LMAX % 16 == 0;
+
 template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool use16, bool PRED = false, bool use8 = false>
-__device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slot, const int d) {
+__device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slot, const int d, T *brow) {
     typedef T v4 __attribute__((ext_vector_type(4)));
     constexpr int LQ = LMAX / 4;
     if (slot >= a.nslots) return;
@@ -147,6 +156,10 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
             for (int r = 0; r < 4; ++r) b4[r] = quad_bcast<2>(m4[r]) + b4[r];
 #pragma unroll
             for (int r = 0; r < 4; ++r) b4[r] = quad_bcast<3>(m4[r]) + b4[r];
+            // every lane of the quad holds the same b4 and stores it (a
+            // store by lane 0 alone would split the unrolled loop into
+            // branches and stop the loads from being issued ahead)
+            if constexpr (kLdsLabels<LABELS, LMAX>) *reinterpret_cast<v4 *>(brow + 4 * j) = b4;
             T bsel = T(0);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -154,13 +167,13 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
                 if (FULL || l < L) {
                     const T m = m4[r];
                     const T b = b4[r];
-                    if (LABELS && r == d) bsel = b;
+                    if (LABELS && !kLdsLabels<LABELS, LMAX> && r == d) bsel = b;
                     const T h = b - m;
                     if (l == 0) { f[0] = h; mu = h; }
                     else { f[l] = tmin(h, f[l - 1] + s); mu = tmin(mu, h); }
                 }
             }
-            if (LABELS && (FULL || 4 * j + d < L)) {
+            if (LABELS && !kLdsLabels<LABELS, LMAX> && (FULL || 4 * j + d < L)) {
                 if (arg < 0 || bsel < best) { best = bsel; arg = 4 * j + d; }
             }
         }
@@ -189,6 +202,19 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
                                                    ors, out_off + (uint32_t)j * a.plane_bytes, 0, GINF_QUAD_STORE_POLICY);
         }
     }
+    if constexpr (kLdsLabels<LABELS, LMAX>) {
+        // lane d: the lowest-index minimum over labels [d LMAX/4, (d+1) LMAX/4)
+        constexpr int R = LMAX / 4;
+#pragma unroll
+        for (int k = 0; k < R / 4; ++k) {
+            const v4 b4 = *reinterpret_cast<const v4 *>(brow + d * R + 4 * k);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int l = d * R + 4 * k + r;
+                if ((FULL || l < L) && (arg < 0 || b4[r] < best)) { best = b4[r]; arg = l; }
+            }
+        }
+    }
     if (LABELS) {
         // Combine the residues: smaller belief wins, ties go to the lower label.
 #pragma unroll
@@ -205,22 +231,31 @@ __device__ __forceinline__ void quad_node_impl(const SweepArgs &a, const int slo
 // The two storage forms are separate straight-line bodies; the uniform flag
 // picks one per wave (a branch inside the unrolled label loop would stop the
 // compiler from issuing the loads ahead).
+// The belief rows of the labelling instances are declared here, once per
+// kernel, and shared by the storage-form bodies; node threadIdx.x / 4 of a
+// block (at most 256 threads, 64 nodes) owns row threadIdx.x / 4.
 template <typename T, int LMAX, bool ZEROIN, bool LABELS, bool FULL, bool D16 = false, bool PRED = false,
           bool D8 = false>
 __device__ __forceinline__ void quad_node(const SweepArgs &a, const int slot, const int d) {
+    T *brow = nullptr;
+    if constexpr (kLdsLabels<LABELS, LMAX>) {
+        // row pitch LMAX + 4 floats: the 16 lane-0 writes of a wave start on distinct bank quads
+        __shared__ __attribute__((aligned(16))) T beliefs[64 * (LMAX + 4)];
+        brow = beliefs + (threadIdx.x >> 2) * (LMAX + 4);
+    }
     if constexpr (D16) {
         if (__builtin_amdgcn_readfirstlane(*a.fit_flag) == 0) {
             if constexpr (D8) {
                 if (a.fit8_flag && __builtin_amdgcn_readfirstlane(*a.fit8_flag) == 0) {
-                    quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, true, PRED, true>(a, slot, d);
+                    quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, true, PRED, true>(a, slot, d, brow);
                     return;
                 }
             }
-            quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, true, PRED>(a, slot, d);
+            quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, true, PRED>(a, slot, d, brow);
             return;
         }
     }
-    quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, false, PRED>(a, slot, d);
+    quad_node_impl<T, LMAX, ZEROIN, LABELS, FULL, false, PRED>(a, slot, d, brow);
 }
 
 }  // namespace
