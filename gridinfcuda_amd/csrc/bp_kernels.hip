@@ -301,7 +301,9 @@ __global__ __launch_bounds__(256) void k_energy_final(const double *partials, in
 // Planes i = blockIdx.z, blockIdx.z + gridDim.z, ... < C: the plan launches
 // one z-block per plane, and the conditional redo one z-block in all, so that
 // the redo's blocks are few when they have nothing to do.
-template <typename T, typename S>
+// VEC (even W, 8-byte aligned source): the two columns of a row are one
+// 8-byte load.
+template <typename T, typename S, bool VEC>
 __device__ __forceinline__ void relayout_quad_pair(const T *__restrict__ src, int W, int H, int L, int C,
                                                    const LevelGeom &g0, S *d00, S *d01, int coarse, const LevelGeom &g1,
                                                    S *d10, S *d11, int k, int Y, int i, bool &bad, T (&v)[4][2][2]) {
@@ -316,8 +318,15 @@ __device__ __forceinline__ void relayout_quad_pair(const T *__restrict__ src, in
             v[q][r][1] = T(0);
             if (l < L && (r == 0 || ey)) {
                 const T *row = src + ((int64_t)l * H + y0 + r) * W;
-                v[q][r][0] = row[x0];
-                if (ex) v[q][r][1] = row[x0 + 1];
+                if constexpr (VEC) {
+                    typedef T t2 __attribute__((ext_vector_type(2)));
+                    const t2 p = *reinterpret_cast<const t2 *>(row + x0);  // W even: x0 + 1 < W
+                    v[q][r][0] = p[0];
+                    v[q][r][1] = p[1];
+                } else {
+                    v[q][r][0] = row[x0];
+                    if (ex) v[q][r][1] = row[x0 + 1];
+                }
             }
         }
     }
@@ -355,7 +364,7 @@ __device__ __forceinline__ void relayout_quad_pair(const T *__restrict__ src, in
     }
 }
 
-template <typename T, typename S>
+template <typename T, typename S, bool VEC>
 __global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src, int W, int H, int L, int C,
                                                        LevelGeom g0, S *d00, S *d01, int coarse, LevelGeom g1, S *d10,
                                                        S *d11, int *flag, const int *only_if) {
@@ -366,7 +375,7 @@ __global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src
     bool bad = false;
     for (int i = blockIdx.z; i < C; i += gridDim.z) {
         T v[4][2][2];
-        relayout_quad_pair<T, S>(src, W, H, L, C, g0, d00, d01, coarse, g1, d10, d11, k, Y, i, bad, v);
+        relayout_quad_pair<T, S, VEC>(src, W, H, L, C, g0, d00, d01, coarse, g1, d10, d11, k, Y, i, bad, v);
     }
     report_bad(bad, flag);
 }
@@ -377,7 +386,7 @@ __global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src
 // of ginf_minpool_costs, without reading the cost volume a second time.  A
 // lane's two row pairs give the minimum over 2 columns x 4 rows; the
 // neighbouring lane (the next column pair) completes the 4 x 4 block.
-template <typename T, typename S>
+template <typename T, typename S, bool VEC>
 __global__ __launch_bounds__(256) void k_relayout_quad_mp(const T *__restrict__ src, int W, int H, int L, int C,
                                                           LevelGeom g0, S *d00, S *d01, int coarse, LevelGeom g1,
                                                           S *d10, S *d11, int *flag, uint16_t *__restrict__ mp, int w4,
@@ -393,7 +402,7 @@ __global__ __launch_bounds__(256) void k_relayout_quad_mp(const T *__restrict__ 
             const int Y = 2 * Y4 + h;
             if (!live || 2 * Y >= H) continue;
             T v[4][2][2];
-            relayout_quad_pair<T, S>(src, W, H, L, C, g0, d00, d01, coarse, g1, d10, d11, k, Y, i, bad, v);
+            relayout_quad_pair<T, S, VEC>(src, W, H, L, C, g0, d00, d01, coarse, g1, d10, d11, k, Y, i, bad, v);
 #pragma unroll
             for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -501,9 +510,17 @@ hipError_t launch_relayout_quad_mp(int dtype, const void *src, int W, int H, int
     const dim3 grid((unsigned)((W + 511) / 512), (unsigned)((H + 3) / 4), (unsigned)ly.C), block(256);
     if (ly.C > 65535 || (H + 3) / 4 > 65535) return hipErrorInvalidValue;
     const int w4 = (W + 3) / 4;
+    const bool vec = W % 2 == 0 && (uintptr_t)src % 8 == 0;
 #define GINF_RQM(TT, SS)                                                                                         \
-    hipLaunchKernelGGL((k_relayout_quad_mp<TT, SS>), grid, block, 0, st, (const TT *)src, W, H, L, ly.C, g0,      \
-                       (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, mp, w4, scale)
+    do if (vec)                                                                                                  \
+        hipLaunchKernelGGL((k_relayout_quad_mp<TT, SS, true>), grid, block, 0, st, (const TT *)src, W, H, L,      \
+                           ly.C, g0, (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, mp, w4, \
+                           scale);                                                                               \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_relayout_quad_mp<TT, SS, false>), grid, block, 0, st, (const TT *)src, W, H, L,     \
+                           ly.C, g0, (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, mp, w4, \
+                           scale);                                                                               \
+    while (0)
     if (data8) { if (dtype == 0) GINF_RQM(float, uint8_t); else GINF_RQM(int, uint8_t); }
     else if (data16) { if (dtype == 0) GINF_RQM(float, uint16_t); else GINF_RQM(int, uint16_t); }
     else { if (dtype == 0) GINF_RQM(float, float); else GINF_RQM(int, int); }
@@ -518,9 +535,17 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
         const dim3 grid((unsigned)((W + 511) / 512), (unsigned)((H + 1) / 2), only_if ? 1u : (unsigned)ly.C),
             block(256);
         if (ly.C > 65535 || (H + 1) / 2 > 65535) return hipErrorInvalidValue;
+        const bool vec = W % 2 == 0 && (uintptr_t)src % 8 == 0;
 #define GINF_RQ(TT, SS)                                                                                          \
-        hipLaunchKernelGGL((k_relayout_quad<TT, SS>), grid, block, 0, st, (const TT *)src, W, H, L, ly.C, g0,     \
-                           (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, only_if)
+        do if (vec)                                                                                              \
+            hipLaunchKernelGGL((k_relayout_quad<TT, SS, true>), grid, block, 0, st, (const TT *)src, W, H, L,     \
+                               ly.C, g0, (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag,    \
+                               only_if);                                                                         \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_relayout_quad<TT, SS, false>), grid, block, 0, st, (const TT *)src, W, H, L,    \
+                               ly.C, g0, (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag,    \
+                               only_if);                                                                         \
+        while (0)
         if (data8) { if (dtype == 0) GINF_RQ(float, uint8_t); else GINF_RQ(int, uint8_t); }
         else if (data16) { if (dtype == 0) GINF_RQ(float, uint16_t); else GINF_RQ(int, uint16_t); }
         else { if (dtype == 0) GINF_RQ(float, float); else GINF_RQ(int, int); }
