@@ -82,6 +82,39 @@ def test_icm_sync_cost_storage(oracle_mod, form):
     np.testing.assert_array_equal(out.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("L,H,W", [(12, 21, 36), (32, 25, 40), (128, 20, 32)])
+@pytest.mark.parametrize("form,dtype", [("fits", np.float32), ("fits", np.int32), ("past255", np.int32),
+                                        ("fraction", np.float32), ("negative", np.int32), ("negative", np.float32),
+                                        ("negzero", np.float32)])
+def test_icm_sync_table_storage(oracle_mod, form, dtype, L, H, W):
+    """Synchronous ICM with four nodes per thread on tables whose entries are
+    small integers, past 255, fractional, negative or -0, with the costs read
+    from their 8-bit copy and in T: the labels equal the oracle's."""
+    torch = torch_cuda()
+    iters = 3
+    say(f"icm sync table storage {form} {np.dtype(dtype).name} L={L} H={H} W={W}")
+    data, table, init = case(17 + L, L, H, W, dtype, "tl")
+    if form == "past255":
+        table[2, 5] = 300
+    elif form == "fraction":
+        table[1, L - 1] += 0.5
+    elif form == "negative":
+        table[L - 1, 0] = -1
+    elif form == "negzero":
+        table[3, 3] = -0.0
+    for cost_form in ("fits", "past255"):
+        d = data.copy()
+        if cost_form == "past255":
+            d[1, 5, 9] = 280
+        want = oracle_mod.icm(d, table, init, iters, sync=True)
+        dd, td, idev = (torch.from_numpy(a).cuda() for a in (d, table, init))
+        out = torch.full((H, W), -1, dtype=torch.int32, device="cuda")
+        G.icm(W, H, L, G.dtype_code(d.dtype), dd.data_ptr(), td.data_ptr(), idev.data_ptr(), iters, G.ICM_SYNC,
+              out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), want)
+
+
 @pytest.mark.parametrize("dtype", [np.int32, np.float32])
 def test_icm_sync_unaligned_buffers(oracle_mod, dtype):
     """W % 4 == 0 but a result buffer 4 bytes past a 16-byte boundary: the
