@@ -364,14 +364,18 @@ __device__ __forceinline__ void relayout_quad_pair(const T *__restrict__ src, in
     }
 }
 
+// Block shape of the quad relayouts (256 threads): 2^txs column pairs of a
+// row pair by 256 >> txs row pairs, txs chosen on the host
+// (relayout_pair_shift) so that narrow grids do not leave most lanes of a
+// block's row idle.
 template <typename T, typename S, bool VEC>
 __global__ __launch_bounds__(256) void k_relayout_quad(const T *__restrict__ src, int W, int H, int L, int C,
                                                        LevelGeom g0, S *d00, S *d01, int coarse, LevelGeom g1, S *d10,
-                                                       S *d11, int *flag, const int *only_if) {
+                                                       S *d11, int *flag, const int *only_if, int txs) {
     if (only_if && *only_if == 0) return;  // conditional redo in T: nothing to redo
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // column pair
-    const int Y = blockIdx.y;
-    if (2 * k >= W) return;
+    const int k = blockIdx.x * (1 << txs) + (threadIdx.x & ((1 << txs) - 1));  // column pair
+    const int Y = blockIdx.y * (256 >> txs) + (threadIdx.x >> txs);           // row pair
+    if (2 * k >= W || 2 * Y >= H) return;
     bool bad = false;
     for (int i = blockIdx.z; i < C; i += gridDim.z) {
         T v[4][2][2];
@@ -390,10 +394,11 @@ template <typename T, typename S, bool VEC>
 __global__ __launch_bounds__(256) void k_relayout_quad_mp(const T *__restrict__ src, int W, int H, int L, int C,
                                                           LevelGeom g0, S *d00, S *d01, int coarse, LevelGeom g1,
                                                           S *d10, S *d11, int *flag, uint16_t *__restrict__ mp, int w4,
-                                                          float scale) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // column pair
-    const int Y4 = blockIdx.y;
-    const bool live = 2 * k < W;  // no early exit: lanes k and k ^ 1 exchange their minima
+                                                          float scale, int txs) {
+    const int k = blockIdx.x * (1 << txs) + (threadIdx.x & ((1 << txs) - 1));  // column pair
+    const int Y4 = blockIdx.y * (256 >> txs) + (threadIdx.x >> txs);          // pair of row pairs
+    // no early exit: lanes k and k ^ 1 (adjacent lanes, 2^txs >= 16) exchange their minima
+    const bool live = 2 * k < W && 4 * Y4 < H;
     bool bad = false;
     for (int i = blockIdx.z; i < C; i += gridDim.z) {
         float mn[4] = {3.4e38f, 3.4e38f, 3.4e38f, 3.4e38f};
@@ -502,24 +507,42 @@ hipError_t launch_relayout(int dtype, const void *src, int W, int H, int L, cons
     return hipGetLastError();
 }
 
+// log2 of the column pairs per block of the quad relayouts: the power of two
+// in [16, 256] that leaves the fewest idle lanes on a row (the larger on a
+// tie), e.g. 64 for W = 1920 or 640 (15 and 5 full blocks instead of 3.75 and
+// 1.25 blocks of 256).
+static int relayout_pair_shift(int W) {
+    const int64_t P = ((int64_t)W + 1) / 2;
+    int best = 8;
+    int64_t waste = -1;
+    for (int s = 8; s >= 4; --s) {
+        const int64_t tx = (int64_t)1 << s, w = (P + tx - 1) / tx * tx - P;
+        if (waste < 0 || w < waste) { best = s; waste = w; }
+    }
+    return best;
+}
+
 hipError_t launch_relayout_quad_mp(int dtype, const void *src, int W, int H, int L, const StateLayout &ly,
                                    const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1,
                                    void *d10, void *d11, hipStream_t st, bool data16, int *flag, uint16_t *mp,
                                    float scale, bool data8) {
     if (!(ly.interleave && ly.G == 4)) return hipErrorNotSupported;
-    const dim3 grid((unsigned)((W + 511) / 512), (unsigned)((H + 3) / 4), (unsigned)ly.C), block(256);
-    if (ly.C > 65535 || (H + 3) / 4 > 65535) return hipErrorInvalidValue;
+    const int txs = relayout_pair_shift(W);
+    const int64_t gx = (((int64_t)W + 1) / 2 + (1 << txs) - 1) >> txs;
+    const int64_t gy = (((int64_t)H + 3) / 4 + (256 >> txs) - 1) / (256 >> txs);
+    if (ly.C > 65535 || gy > 65535 || gx > 0x7fffffff) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)ly.C), block(256);
     const int w4 = (W + 3) / 4;
     const bool vec = W % 2 == 0 && (uintptr_t)src % 8 == 0;
 #define GINF_RQM(TT, SS)                                                                                         \
     do if (vec)                                                                                                  \
         hipLaunchKernelGGL((k_relayout_quad_mp<TT, SS, true>), grid, block, 0, st, (const TT *)src, W, H, L,      \
                            ly.C, g0, (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, mp, w4, \
-                           scale);                                                                               \
+                           scale, txs);                                                                          \
     else                                                                                                         \
         hipLaunchKernelGGL((k_relayout_quad_mp<TT, SS, false>), grid, block, 0, st, (const TT *)src, W, H, L,     \
                            ly.C, g0, (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag, mp, w4, \
-                           scale);                                                                               \
+                           scale, txs);                                                                          \
     while (0)
     if (data8) { if (dtype == 0) GINF_RQM(float, uint8_t); else GINF_RQM(int, uint8_t); }
     else if (data16) { if (dtype == 0) GINF_RQM(float, uint16_t); else GINF_RQM(int, uint16_t); }
@@ -532,19 +555,21 @@ hipError_t launch_relayout_pyr(int dtype, const void *src, int W, int H, int L, 
                                const LevelGeom &g0, void *d00, void *d01, bool coarse, const LevelGeom &g1, void *d10,
                                void *d11, hipStream_t st, bool data16, int *flag, const int *only_if, bool data8) {
     if (ly.interleave && ly.G == 4) {  // label quads: register-only relayout
-        const dim3 grid((unsigned)((W + 511) / 512), (unsigned)((H + 1) / 2), only_if ? 1u : (unsigned)ly.C),
-            block(256);
-        if (ly.C > 65535 || (H + 1) / 2 > 65535) return hipErrorInvalidValue;
+        const int txs = relayout_pair_shift(W);
+        const int64_t gx = (((int64_t)W + 1) / 2 + (1 << txs) - 1) >> txs;
+        const int64_t gy = (((int64_t)H + 1) / 2 + (256 >> txs) - 1) / (256 >> txs);
+        if (ly.C > 65535 || gy > 65535 || gx > 0x7fffffff) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)gx, (unsigned)gy, only_if ? 1u : (unsigned)ly.C), block(256);
         const bool vec = W % 2 == 0 && (uintptr_t)src % 8 == 0;
 #define GINF_RQ(TT, SS)                                                                                          \
         do if (vec)                                                                                              \
             hipLaunchKernelGGL((k_relayout_quad<TT, SS, true>), grid, block, 0, st, (const TT *)src, W, H, L,     \
                                ly.C, g0, (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag,    \
-                               only_if);                                                                         \
+                               only_if, txs);                                                                    \
         else                                                                                                     \
             hipLaunchKernelGGL((k_relayout_quad<TT, SS, false>), grid, block, 0, st, (const TT *)src, W, H, L,    \
                                ly.C, g0, (SS *)d00, (SS *)d01, coarse ? 1 : 0, g1, (SS *)d10, (SS *)d11, flag,    \
-                               only_if);                                                                         \
+                               only_if, txs);                                                                    \
         while (0)
         if (data8) { if (dtype == 0) GINF_RQ(float, uint8_t); else GINF_RQ(int, uint8_t); }
         else if (data16) { if (dtype == 0) GINF_RQ(float, uint16_t); else GINF_RQ(int, uint16_t); }
